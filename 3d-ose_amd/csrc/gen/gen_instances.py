@@ -1,0 +1,92 @@
+"""Generate the per-extent kernel instances of the HIP backend.
+
+For every supported FFT extent N = N1 * N2 this writes inst_<N>.hip, which
+instantiates the strided (y/z) and x-line kernels of ose_kernels.h for that
+split and fills an ose::FftEntry with their addresses, block sizes and LDS
+sizes.  It also writes dft_gen.h (the register DFTs for every N1, N2) and
+ose_sizes.inc (the registry list).
+
+Supported extents are the even 2,3,5,7-smooth sizes the configs need (plus
+152 = 8*19, the reference's own 101+25+26 grid).  Extents outside the list
+are rejected by ose_add_model with OSE_EUNSUPPORTED; the host's extent
+chooser rounds a padded size up to the next supported one.
+
+Run: python gen_instances.py <outdir>
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+# N: (N1, N2).  N1 = register DFT length of step A (run by N2 threads per
+# pencil), N2 = register DFT length of step B (run by N1 threads).
+SIZES = {
+    16: (4, 4), 24: (4, 6), 32: (4, 8), 48: (6, 8), 64: (8, 8), 80: (8, 10),
+    96: (8, 12), 112: (8, 14), 128: (8, 16), 140: (10, 14), 144: (12, 12),
+    152: (8, 19), 160: (10, 16), 192: (12, 16), 200: (10, 20), 224: (14, 16),
+    240: (15, 16), 256: (16, 16), 280: (14, 20), 288: (16, 18), 320: (16, 20),
+    336: (16, 21), 378: (18, 21), 384: (16, 24), 400: (20, 20), 448: (16, 28),
+    480: (20, 24), 512: (16, 32), 576: (24, 24), 600: (20, 30), 640: (20, 32),
+    672: (21, 32), 720: (24, 30), 768: (24, 32), 784: (28, 28), 800: (25, 32),
+    864: (27, 32), 960: (30, 32), 1024: (32, 32),
+}
+
+STRIDED_T = 16
+
+
+def x_lines(nt):
+    return max(1, 256 // nt)
+
+
+INST = """// Generated by csrc/gen/gen_instances.py -- do not edit.
+#include "ose_kernels.h"
+#include "ose_entry.h"
+
+namespace ose {{
+void fill_entry_{N}(FftEntry& e)
+{{
+    constexpr int N1 = {N1}, N2 = {N2}, T = {T}, P = {P};
+    using SC = StridedCfg<N1, N2, T>;
+    using XC = XCfg<N1, N2, P>;
+    e.N = {N}; e.N1 = N1; e.N2 = N2;
+    e.s_T = T; e.s_threads = SC::THREADS; e.s_lds = SC::LDS_BYTES;
+    e.ystr[0] = (const void*)&k_strided<N1, N2, T, -1>;
+    e.ystr[1] = (const void*)&k_strided<N1, N2, T, 1>;
+    e.zsym = (const void*)&k_zsym<N1, N2, T>;
+    e.x_P = P; e.x_threads = XC::THREADS; e.x_lds = XC::LDS_BYTES;
+    e.xk[XM_ORDER0] = (const void*)&k_xpass<N1, N2, P, XM_ORDER0>;
+    e.xk[XM_MID] = (const void*)&k_xpass<N1, N2, P, XM_MID>;
+    e.xk[XM_LAST] = (const void*)&k_xpass<N1, N2, P, XM_LAST>;
+    e.xk[XM_FWD] = (const void*)&k_xpass<N1, N2, P, XM_FWD>;
+    e.xk[XM_INV] = (const void*)&k_xpass<N1, N2, P, XM_INV>;
+}}
+}}  // namespace ose
+"""
+
+
+def main():
+    out = sys.argv[1]
+    only = [int(a) for a in sys.argv[2:]]
+    sizes = {n: SIZES[n] for n in (only or SIZES)}
+    os.makedirs(out, exist_ok=True)
+    factors = sorted({f for (a, b) in SIZES.values() for f in (a, b)})
+    subprocess.check_call([sys.executable, os.path.join(HERE, "gen_dft.py"),
+                           os.path.join(out, "dft_gen.h")] + [str(f) for f in factors])
+    with open(os.path.join(out, "ose_sizes.inc"), "w") as f:
+        f.write("// Generated by csrc/gen/gen_instances.py -- do not edit.\n")
+        for n in sorted(sizes):
+            f.write(f"OSE_SIZE({n})\n")
+    for n in sorted(sizes):
+        n1, n2 = sizes[n]
+        nt = max(n1, n2)
+        src = INST.format(N=n, N1=n1, N2=n2, T=STRIDED_T, P=x_lines(nt))
+        path = os.path.join(out, f"inst_{n}.hip")
+        old = open(path).read() if os.path.exists(path) else None
+        if old != src:
+            with open(path, "w") as f:
+                f.write(src)
+
+
+if __name__ == "__main__":
+    main()

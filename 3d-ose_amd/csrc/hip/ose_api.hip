@@ -1,0 +1,978 @@
+// ose_api.hip -- C-ABI implementation of the HIP backend (include/ose_api.h).
+//
+// A context owns one HIP stream and, per model, the velocity (as v/R), the
+// taper profiles, the symbol tables and the twiddle tables, and per shot
+// batch the four complex fields of the fused step:
+//     X0, X1  x-spectral Chebyshev terms (ping-pong slots)
+//     S       spectrum scratch of passes 1-3
+//     A       accumulator (physical space)
+// Shots of one model are batched into one launch per pass (blockIdx.z /
+// blockIdx.y = shot), so a model's step is 4K + 1 launches and one step of
+// the whole context adds a counter increment.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../../include/ose_api.h"
+#include "ose_entry.h"
+#include "ose_kernels.h"
+
+namespace ose {
+
+// ---------------------------------------------------------------------
+// non-template kernels
+// ---------------------------------------------------------------------
+__global__ void k_inject(InjectArgs a)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int shot = blockIdx.y;
+    if (k >= a.nx) return;
+    const int it = *a.it;
+    if (it >= a.nt) return;
+    const float s = a.s[(long)shot * a.nt + it];
+    const float h = a.h[(long)shot * a.nt + it];
+    const int sx = a.src_x[shot];
+    const float2 e = a.tw[(int)(((long)k * sx) % a.nx)];
+    float2* p = a.X0 + (long)shot * a.shot_stride + a.src_line[shot] + k;
+    const float2 o = *p;
+    *p = make_float2(o.x + s * e.x - h * e.y, o.y + s * e.y + h * e.x);
+}
+
+__global__ void k_advance(int* it)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *it = *it + 1;
+}
+
+__global__ void k_scale_vr(float2* f, const float* vr, long n, long shot_stride, float s)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float2* p = f + (long)blockIdx.y * shot_stride + i;
+    const float m = vr[i] * s;
+    *p = make_float2(p->x * m, p->y * m);
+}
+
+#define OSE_SIZE(N) void fill_entry_##N(FftEntry&);
+#include "ose_sizes.inc"
+#undef OSE_SIZE
+
+namespace {
+struct Registry {
+    std::vector<FftEntry> entries;
+    Registry()
+    {
+#define OSE_SIZE(N) { FftEntry e; fill_entry_##N(e); entries.push_back(e); }
+#include "ose_sizes.inc"
+#undef OSE_SIZE
+    }
+};
+Registry& registry()
+{
+    static Registry r;
+    return r;
+}
+}  // namespace
+
+const FftEntry* find_entry(int N)
+{
+    for (auto& e : registry().entries)
+        if (e.N == N) return &e;
+    return nullptr;
+}
+
+int next_supported(int n)
+{
+    int best = 0;
+    for (auto& e : registry().entries)
+        if (e.N >= n && (best == 0 || e.N < best)) best = e.N;
+    return best;
+}
+
+}  // namespace ose
+
+using namespace ose;
+
+// ---------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------
+namespace {
+
+enum PassKind { PK_YF = 0, PK_Z = 1, PK_YI = 2, PK_XMID = 3, PK_X0 = 4, PK_XLAST = 5, PK_INJ = 6, PK_N = 7 };
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+struct Model {
+    ose_model_desc d{};
+    std::vector<double> coef;
+    std::vector<float> gx, gy, gz;
+    const FftEntry *ex = nullptr, *ey = nullptr, *ez = nullptr;
+    long ncell = 0;
+    DevBuf vr, g, k2, tw;          // g: gx|gy|gz, k2: kx2|ky2|kz2, tw: twx|twy|twz
+    float* d_vr = nullptr;
+    float *d_gx = nullptr, *d_gy = nullptr, *d_gz = nullptr;
+    float *d_kx2 = nullptr, *d_ky2 = nullptr, *d_kz2 = nullptr;
+    float2 *d_twx = nullptr, *d_twy = nullptr, *d_twz = nullptr;
+    // shot batch
+    std::vector<int> shots;        // context shot ids
+    int nt_cap = 0;
+    DevBuf fields, traces, src, wav;
+    float2 *X[2] = {nullptr, nullptr}, *S = nullptr, *A = nullptr;
+    float* d_traces = nullptr;
+    long* d_src_line = nullptr;
+    int* d_src_x = nullptr;
+    float *d_s = nullptr, *d_h = nullptr;
+    int rec_ny = 0, rec_nx = 0;
+};
+
+struct Shot {
+    int model = 0;
+    int slot = 0;
+    int sx = 0, sy = 0, sz = 0;
+    std::vector<float> s, h;
+    int nt = 0;
+};
+
+struct Pending {
+    int kind;
+    hipEvent_t e0, e1;
+    double bytes;
+};
+
+}  // namespace
+
+struct ose_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<std::unique_ptr<Model>> models;
+    std::vector<Shot> shots;
+    DevBuf it_buf;
+    int* d_it = nullptr;
+    bool finalized = false;
+    bool use_graph = true;
+    bool profile = false;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    std::string err;
+    // profiling
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<Pending> pending;
+    double st_ms[PK_N] = {};
+    long st_n[PK_N] = {};
+    double st_bytes[PK_N] = {};
+
+    int fail(int code, const char* fmt, ...)
+    {
+        char buf[1024];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        err = buf;
+        return code;
+    }
+};
+
+#define HIPCHK(ctx, call)                                                                       \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return (ctx)->fail(OSE_EDEVICE, "%s failed: %s", #call, hipGetErrorString(e_));    \
+    } while (0)
+
+namespace {
+
+int dev_alloc(ose_ctx* c, DevBuf& b, size_t bytes)
+{
+    if (b.p) { (void)hipFree(b.p); b.p = nullptr; }
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        (void)hipGetLastError();
+        return c->fail(OSE_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    return OSE_OK;
+}
+
+int set_device(ose_ctx* c)
+{
+    HIPCHK(c, hipSetDevice(c->device));
+    return OSE_OK;
+}
+
+// root table exp(-2 pi i j / n), FP64 then rounded
+void roots(int n, std::vector<float2>& out)
+{
+    out.resize(n);
+    for (int j = 0; j < n; ++j) {
+        const double a = -2.0 * M_PI * (double)j / (double)n;
+        out[j] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+}
+
+// squared folded wavenumbers (scale_kxkz_3D, ose_3d_kernel.cu:109-123)
+void k2table(int n, double d, std::vector<float>& out)
+{
+    out.resize(n);
+    const double dk = 2.0 * M_PI / (d * (double)n);
+    for (int i = 0; i < n; ++i) {
+        const double k = dk * (double)(i <= n / 2 ? i : n - i);
+        out[i] = (float)(k * k);
+    }
+}
+
+hipEvent_t take_event(ose_ctx* c)
+{
+    if (!c->ev_pool.empty()) {
+        hipEvent_t e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+int launch(ose_ctx* c, int kind, double bytes, const void* fn, dim3 grid, dim3 block,
+           size_t lds, void* args)
+{
+    void* params[] = {args};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->profile && kind >= 0) {
+        e0 = take_event(c);
+        e1 = take_event(c);
+        if (e0) (void)hipEventRecord(e0, c->stream);
+    }
+    hipError_t e = hipLaunchKernel(fn, grid, block, params, lds, c->stream);
+    if (e != hipSuccess)
+        return c->fail(OSE_EDEVICE, "kernel launch (pass %d) failed: %s", kind, hipGetErrorString(e));
+    if (c->profile && e0 && e1) {
+        (void)hipEventRecord(e1, c->stream);
+        c->pending.push_back({kind, e0, e1, bytes});
+    }
+    return OSE_OK;
+}
+
+int collect(ose_ctx* c)
+{
+    for (auto& p : c->pending) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventSynchronize(p.e1));
+        HIPCHK(c, hipEventElapsedTime(&ms, p.e0, p.e1));
+        c->st_ms[p.kind] += ms;
+        c->st_n[p.kind] += 1;
+        c->st_bytes[p.kind] = p.bytes;
+        c->ev_pool.push_back(p.e0);
+        c->ev_pool.push_back(p.e1);
+    }
+    c->pending.clear();
+    return OSE_OK;
+}
+
+// strided pass over the y axis (es = nx) or the z axis (es = nx*ny)
+int launch_strided(ose_ctx* c, int kind, const FftEntry* e, const void* fn, const float2* in,
+                   float2* out, int nx, int ny, int nz, bool zaxis, int nshot, long shot_stride,
+                   const float2* tw, const Model* m, double bytes)
+{
+    StridedArgs a{};
+    a.in = in;
+    a.out = out;
+    a.ncol = nx;
+    a.es = zaxis ? (long)nx * ny : (long)nx;
+    a.os = zaxis ? (long)nx : (long)nx * ny;
+    a.shot_stride = shot_stride;
+    a.tw = tw;
+    if (m) {
+        a.kx2 = m->d_kx2;
+        a.ky2 = m->d_ky2;
+        a.kz2 = m->d_kz2;
+        a.invN = (float)(1.0 / ((double)nx * ny * nz));
+    }
+    dim3 grid((nx + e->s_T - 1) / e->s_T, zaxis ? ny : nz, nshot);
+    return launch(c, kind, bytes, fn, grid, dim3(e->s_threads), e->s_lds, &a);
+}
+
+int launch_x(ose_ctx* c, int kind, const FftEntry* e, int mode, XArgs& a, int nlines, int nshot,
+             double bytes)
+{
+    a.nlines = nlines;
+    dim3 grid((nlines + e->x_P - 1) / e->x_P, nshot);
+    return launch(c, kind, bytes, e->xk[mode], grid, dim3(e->x_threads), e->x_lds, &a);
+}
+
+int prepare_kernels(ose_ctx* c, const FftEntry* e)
+{
+    // allow > 64 KiB of dynamic LDS where a split needs it
+    const void* fs[] = {e->ystr[0], e->ystr[1], e->zsym};
+    for (auto f : fs)
+        if (e->s_lds > 65536)
+            HIPCHK(c, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->s_lds));
+    for (auto f : e->xk)
+        if (e->x_lds > 65536)
+            HIPCHK(c, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->x_lds));
+    return OSE_OK;
+}
+
+XArgs base_xargs(const Model& m, int nshot)
+{
+    XArgs a{};
+    a.vr = m.d_vr;
+    a.gx = m.d_gx;
+    a.gy = m.d_gy;
+    a.gz = m.d_gz;
+    a.tw = m.d_twx;
+    a.shot_stride = m.ncell;
+    a.ny = m.d.nyb;
+    a.inv_nx = (float)(1.0 / (double)m.d.nxb);
+    (void)nshot;
+    return a;
+}
+
+// one OSE step of every shot of model m (the body of wavefield_extro_ori2
+// plus injection and receiver gather)
+int enqueue_model_step(ose_ctx* c, Model& m, int* d_it)
+{
+    const int nshot = (int)m.shots.size();
+    if (nshot == 0) return OSE_OK;
+    const int nx = m.d.nxb, ny = m.d.nyb, nz = m.d.nzb;
+    const long cells = m.ncell * nshot;
+    int rc;
+    // source injection into X0 (x-spectral)
+    {
+        InjectArgs a{};
+        a.X0 = m.X[0];
+        a.shot_stride = m.ncell;
+        a.nx = nx;
+        a.src_line = m.d_src_line;
+        a.src_x = m.d_src_x;
+        a.s = m.d_s;
+        a.h = m.d_h;
+        a.nt = m.nt_cap;
+        a.it = d_it;
+        a.tw = m.d_twx;
+        if ((rc = launch(c, PK_INJ, 16.0 * nx * nshot, (const void*)&k_inject,
+                         dim3((nx + 63) / 64, nshot), dim3(64), 0, &a)))
+            return rc;
+    }
+    const int K = m.d.K;
+    for (int j = 0; j < K; ++j) {
+        float2* Xj = m.X[j % 2];
+        if ((rc = launch_strided(c, PK_YF, m.ey, m.ey->ystr[0], Xj, m.S, nx, ny, nz, false, nshot,
+                                 m.ncell, m.d_twy, nullptr, 16.0 * cells)))
+            return rc;
+        if ((rc = launch_strided(c, PK_Z, m.ez, m.ez->zsym, m.S, m.S, nx, ny, nz, true, nshot,
+                                 m.ncell, m.d_twz, &m, 16.0 * cells)))
+            return rc;
+        if ((rc = launch_strided(c, PK_YI, m.ey, m.ey->ystr[1], m.S, m.S, nx, ny, nz, false, nshot,
+                                 m.ncell, m.d_twy, nullptr, 16.0 * cells)))
+            return rc;
+        XArgs a = base_xargs(m, nshot);
+        a.S = m.S;
+        int mode, kind;
+        double bytes;
+        if (j == 0) {
+            mode = XM_ORDER0;
+            kind = PK_X0;
+            a.Xprev = m.X[0];
+            a.Xout = m.X[1];
+            a.c0 = (float)m.coef[0];
+            a.c1 = (float)m.coef[1];
+            bytes = 36.0 * cells;   // S, X0, vr read; X1, A written
+        } else {
+            a.Xprev = m.X[(j - 1) % 2];
+            a.c1 = (float)m.coef[j + 1];
+            if (j == K - 1) {
+                mode = XM_LAST;
+                kind = PK_XLAST;
+                a.Xout = m.X[0];
+                a.traces = m.d_traces;
+                a.trace_shot_stride = (long)m.nt_cap * m.rec_ny * m.rec_nx;
+                a.rec_z = m.d.bu + m.d.rec_z;
+                a.rec_y0 = m.d.bf;
+                a.rec_ny = m.rec_ny;
+                a.rec_x0 = m.d.bl;
+                a.rec_nx = m.rec_nx;
+                a.it = d_it;
+                a.nt_cap = m.nt_cap;
+                bytes = 36.0 * cells;   // S, Xprev, vr, A read; X0 written
+            } else {
+                mode = XM_MID;
+                kind = PK_XMID;
+                a.Xout = m.X[(j + 1) % 2];
+                bytes = 44.0 * cells;   // S, Xprev, vr, A read; X', A written
+            }
+        }
+        if ((rc = launch_x(c, kind, m.ex, mode, a, ny * nz, nshot, bytes))) return rc;
+    }
+    return OSE_OK;
+}
+
+int enqueue_step(ose_ctx* c)
+{
+    int rc;
+    for (auto& m : c->models)
+        if ((rc = enqueue_model_step(c, *m, c->d_it))) return rc;
+    return launch(c, -1, 0.0, (const void*)&k_advance, dim3(1), dim3(64), 0, &c->d_it);
+}
+
+int finalize(ose_ctx* c)
+{
+    if (c->finalized) return OSE_OK;
+    int rc;
+    if ((rc = set_device(c))) return rc;
+    for (auto& mp : c->models) {
+        Model& m = *mp;
+        const int nshot = (int)m.shots.size();
+        if (nshot == 0) continue;
+        m.nt_cap = 0;
+        for (int id : m.shots) m.nt_cap = std::max(m.nt_cap, c->shots[id].nt);
+        m.rec_ny = m.d.ny;
+        m.rec_nx = m.d.nx;
+        const size_t fbytes = sizeof(float2) * (size_t)m.ncell * nshot;
+        if ((rc = dev_alloc(c, m.fields, 4 * fbytes))) return rc;
+        char* base = (char*)m.fields.p;
+        m.X[0] = (float2*)base;
+        m.X[1] = (float2*)(base + fbytes);
+        m.S = (float2*)(base + 2 * fbytes);
+        m.A = (float2*)(base + 3 * fbytes);
+        HIPCHK(c, hipMemsetAsync(m.fields.p, 0, 4 * fbytes, c->stream));
+        const size_t tbytes = sizeof(float) * (size_t)m.nt_cap * m.rec_ny * m.rec_nx * nshot;
+        if ((rc = dev_alloc(c, m.traces, tbytes))) return rc;
+        m.d_traces = (float*)m.traces.p;
+        HIPCHK(c, hipMemsetAsync(m.traces.p, 0, tbytes, c->stream));
+        // sources
+        std::vector<long> line(nshot);
+        std::vector<int> sx(nshot);
+        std::vector<float> s((size_t)nshot * m.nt_cap, 0.f), h((size_t)nshot * m.nt_cap, 0.f);
+        for (int k = 0; k < nshot; ++k) {
+            const Shot& sh = c->shots[m.shots[k]];
+            line[k] = ((long)sh.sz * m.d.nyb + sh.sy) * m.d.nxb;
+            sx[k] = sh.sx;
+            std::copy(sh.s.begin(), sh.s.end(), s.begin() + (size_t)k * m.nt_cap);
+            std::copy(sh.h.begin(), sh.h.end(), h.begin() + (size_t)k * m.nt_cap);
+        }
+        const size_t sb = sizeof(long) * nshot + sizeof(int) * nshot;
+        if ((rc = dev_alloc(c, m.src, sb + 64))) return rc;
+        m.d_src_line = (long*)m.src.p;
+        m.d_src_x = (int*)((char*)m.src.p + sizeof(long) * nshot);
+        HIPCHK(c, hipMemcpy(m.d_src_line, line.data(), sizeof(long) * nshot, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(m.d_src_x, sx.data(), sizeof(int) * nshot, hipMemcpyHostToDevice));
+        const size_t wb = sizeof(float) * (size_t)nshot * m.nt_cap;
+        if ((rc = dev_alloc(c, m.wav, 2 * wb))) return rc;
+        m.d_s = (float*)m.wav.p;
+        m.d_h = (float*)((char*)m.wav.p + wb);
+        if (wb) {
+            HIPCHK(c, hipMemcpy(m.d_s, s.data(), wb, hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(m.d_h, h.data(), wb, hipMemcpyHostToDevice));
+        }
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->finalized = true;
+    return OSE_OK;
+}
+
+int build_graph(ose_ctx* c)
+{
+    if (c->gexec) return OSE_OK;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_step(c);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return c->fail(OSE_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        c->gexec = nullptr;
+        return c->fail(OSE_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    c->graph = g;
+    return OSE_OK;
+}
+
+void drop_graph(ose_ctx* c)
+{
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+}
+
+Shot* get_shot(ose_ctx* c, int id)
+{
+    if (id < 0 || id >= (int)c->shots.size()) return nullptr;
+    return &c->shots[id];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------
+// extern "C"
+// ---------------------------------------------------------------------
+extern "C" {
+
+int ose_create(ose_ctx** out, int device)
+{
+    if (!out) return OSE_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return OSE_EDEVICE;
+    if (device < 0 || device >= n) return OSE_EINVAL;
+    auto* c = new (std::nothrow) ose_ctx();
+    if (!c) return OSE_ENOMEM;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return OSE_EDEVICE;
+    }
+    if (dev_alloc(c, c->it_buf, sizeof(int) * 64)) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return OSE_ENOMEM;
+    }
+    c->d_it = (int*)c->it_buf.p;
+    if (hipMemset(c->d_it, 0, sizeof(int)) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return OSE_EDEVICE;
+    }
+    *out = c;
+    return OSE_OK;
+}
+
+int ose_destroy(ose_ctx* c)
+{
+    if (!c) return OSE_EINVAL;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graph(c);
+    for (auto& p : c->pending) {
+        c->ev_pool.push_back(p.e0);
+        c->ev_pool.push_back(p.e1);
+    }
+    for (auto e : c->ev_pool)
+        if (e) (void)hipEventDestroy(e);
+    c->models.clear();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return OSE_OK;
+}
+
+const char* ose_last_error(const ose_ctx* c)
+{
+    if (!c) return "null context";
+    return c->err.c_str();
+}
+
+int ose_supported_extent(int n) { return next_supported(n); }
+int ose_extent_is_supported(int n) { return find_entry(n) != nullptr; }
+
+int ose_add_model(ose_ctx* c, const ose_model_desc* d, const float* v, int* model_id)
+{
+    if (!c) return OSE_EINVAL;
+    if (!d || !v || !d->coef || !d->taper_x || !d->taper_y || !d->taper_z)
+        return c->fail(OSE_EINVAL, "ose_add_model: null argument");
+    if (c->finalized) return c->fail(OSE_ESTATE, "ose_add_model: context already running");
+    if (d->nxb <= 0 || d->nyb <= 0 || d->nzb <= 0 || d->nx <= 0 || d->ny <= 0 || d->nz <= 0 ||
+        d->bl < 0 || d->bf < 0 || d->bu < 0 || d->bl + d->nx > d->nxb || d->bf + d->ny > d->nyb ||
+        d->bu + d->nz > d->nzb)
+        return c->fail(OSE_EINVAL, "ose_add_model: inconsistent extents");
+    if (d->K < 2 || d->K > 64) return c->fail(OSE_EINVAL, "ose_add_model: K must be in [2, 64]");
+    if (!(d->R > 0) || !(d->dx > 0) || !(d->dy > 0) || !(d->dz > 0) || !(d->dt > 0))
+        return c->fail(OSE_EINVAL, "ose_add_model: R, dx, dy, dz, dt must be positive");
+    if (d->rec_z < 0 || d->rec_z >= d->nz) return c->fail(OSE_EINVAL, "ose_add_model: rec_z outside the interior");
+    auto m = std::make_unique<Model>();
+    m->ex = find_entry(d->nxb);
+    m->ey = find_entry(d->nyb);
+    m->ez = find_entry(d->nzb);
+    if (!m->ex || !m->ey || !m->ez)
+        return c->fail(OSE_EUNSUPPORTED, "ose_add_model: extent %dx%dx%d not compiled (next supported: %d %d %d)",
+                       d->nxb, d->nyb, d->nzb, next_supported(d->nxb), next_supported(d->nyb),
+                       next_supported(d->nzb));
+    m->d = *d;
+    m->coef.assign(d->coef, d->coef + d->K + 1);
+    for (double x : m->coef)
+        if (!std::isfinite(x)) return c->fail(OSE_ENUMERIC, "ose_add_model: non-finite coefficient");
+    m->gx.assign(d->taper_x, d->taper_x + d->nxb);
+    m->gy.assign(d->taper_y, d->taper_y + d->nyb);
+    m->gz.assign(d->taper_z, d->taper_z + d->nzb);
+    m->d.coef = nullptr;
+    m->d.taper_x = m->d.taper_y = m->d.taper_z = nullptr;
+    m->ncell = (long)d->nxb * d->nyb * d->nzb;
+    int rc;
+    if ((rc = set_device(c))) return rc;
+    for (const FftEntry* e : {m->ex, m->ey, m->ez})
+        if ((rc = prepare_kernels(c, e))) return rc;
+    // v / R
+    std::vector<float> vr((size_t)m->ncell);
+    for (long i = 0; i < m->ncell; ++i) {
+        if (!std::isfinite(v[i]) || v[i] <= 0.f)
+            return c->fail(OSE_ENUMERIC, "ose_add_model: velocity not finite and positive at cell %ld", i);
+        vr[i] = (float)((double)v[i] / d->R);
+    }
+    if ((rc = dev_alloc(c, m->vr, sizeof(float) * vr.size()))) return rc;
+    m->d_vr = (float*)m->vr.p;
+    HIPCHK(c, hipMemcpy(m->d_vr, vr.data(), sizeof(float) * vr.size(), hipMemcpyHostToDevice));
+    const int nx = d->nxb, ny = d->nyb, nz = d->nzb;
+    if ((rc = dev_alloc(c, m->g, sizeof(float) * (nx + ny + nz)))) return rc;
+    m->d_gx = (float*)m->g.p;
+    m->d_gy = m->d_gx + nx;
+    m->d_gz = m->d_gy + ny;
+    HIPCHK(c, hipMemcpy(m->d_gx, m->gx.data(), sizeof(float) * nx, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(m->d_gy, m->gy.data(), sizeof(float) * ny, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(m->d_gz, m->gz.data(), sizeof(float) * nz, hipMemcpyHostToDevice));
+    std::vector<float> kx, ky, kz;
+    k2table(nx, d->dx, kx);
+    k2table(ny, d->dy, ky);
+    k2table(nz, d->dz, kz);
+    if ((rc = dev_alloc(c, m->k2, sizeof(float) * (nx + ny + nz)))) return rc;
+    m->d_kx2 = (float*)m->k2.p;
+    m->d_ky2 = m->d_kx2 + nx;
+    m->d_kz2 = m->d_ky2 + ny;
+    HIPCHK(c, hipMemcpy(m->d_kx2, kx.data(), sizeof(float) * nx, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(m->d_ky2, ky.data(), sizeof(float) * ny, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(m->d_kz2, kz.data(), sizeof(float) * nz, hipMemcpyHostToDevice));
+    std::vector<float2> tx, ty, tz;
+    roots(nx, tx);
+    roots(ny, ty);
+    roots(nz, tz);
+    if ((rc = dev_alloc(c, m->tw, sizeof(float2) * (nx + ny + nz)))) return rc;
+    m->d_twx = (float2*)m->tw.p;
+    m->d_twy = m->d_twx + nx;
+    m->d_twz = m->d_twy + ny;
+    HIPCHK(c, hipMemcpy(m->d_twx, tx.data(), sizeof(float2) * nx, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(m->d_twy, ty.data(), sizeof(float2) * ny, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(m->d_twz, tz.data(), sizeof(float2) * nz, hipMemcpyHostToDevice));
+    c->models.push_back(std::move(m));
+    if (model_id) *model_id = (int)c->models.size() - 1;
+    return OSE_OK;
+}
+
+int ose_add_shot(ose_ctx* c, int model_id, int sx, int sy, int sz, const float* s, const float* h,
+                 int nt, int* shot_id)
+{
+    if (!c) return OSE_EINVAL;
+    if (c->finalized) return c->fail(OSE_ESTATE, "ose_add_shot: context already running");
+    if (model_id < 0 || model_id >= (int)c->models.size()) return c->fail(OSE_EINVAL, "ose_add_shot: bad model id");
+    Model& m = *c->models[model_id];
+    if (sx < 0 || sx >= m.d.nxb || sy < 0 || sy >= m.d.nyb || sz < 0 || sz >= m.d.nzb)
+        return c->fail(OSE_EINVAL, "ose_add_shot: source outside the grid");
+    if (nt < 0 || (nt > 0 && (!s || !h))) return c->fail(OSE_EINVAL, "ose_add_shot: bad wavelet");
+    Shot sh;
+    sh.model = model_id;
+    sh.slot = (int)m.shots.size();
+    sh.sx = sx;
+    sh.sy = sy;
+    sh.sz = sz;
+    sh.nt = nt;
+    if (nt > 0) {
+        sh.s.assign(s, s + nt);
+        sh.h.assign(h, h + nt);
+        for (int i = 0; i < nt; ++i)
+            if (!std::isfinite(s[i]) || !std::isfinite(h[i]))
+                return c->fail(OSE_ENUMERIC, "ose_add_shot: non-finite wavelet sample %d", i);
+    }
+    c->shots.push_back(sh);
+    m.shots.push_back((int)c->shots.size() - 1);
+    if (shot_id) *shot_id = (int)c->shots.size() - 1;
+    return OSE_OK;
+}
+
+int ose_set_options(ose_ctx* c, int use_graph, int profile)
+{
+    if (!c) return OSE_EINVAL;
+    c->use_graph = use_graph != 0;
+    c->profile = profile != 0;
+    return OSE_OK;
+}
+
+int ose_run(ose_ctx* c, int nsteps)
+{
+    if (!c) return OSE_EINVAL;
+    if (nsteps < 0) return c->fail(OSE_EINVAL, "ose_run: negative step count");
+    if (c->shots.empty()) return c->fail(OSE_ESTATE, "ose_run: no shots");
+    int rc;
+    if ((rc = set_device(c))) return rc;
+    if ((rc = finalize(c))) return rc;
+    const bool graph = c->use_graph && !c->profile;
+    if (graph && (rc = build_graph(c))) {
+        // capture is not available: run eagerly
+        (void)hipGetLastError();
+        c->use_graph = false;
+        drop_graph(c);
+    }
+    for (int s = 0; s < nsteps; ++s) {
+        if (c->use_graph && !c->profile && c->gexec) {
+            HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+        } else if ((rc = enqueue_step(c))) {
+            return rc;
+        }
+    }
+    return OSE_OK;
+}
+
+int ose_sync(ose_ctx* c)
+{
+    if (!c) return OSE_EINVAL;
+    int rc;
+    if ((rc = set_device(c))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return collect(c);
+}
+
+int ose_timed_run(ose_ctx* c, int nsteps, double* ms)
+{
+    if (!c) return OSE_EINVAL;
+    int rc;
+    if ((rc = set_device(c))) return rc;
+    if ((rc = finalize(c))) return rc;
+    if (c->use_graph && !c->profile && (rc = build_graph(c))) {
+        (void)hipGetLastError();
+        c->use_graph = false;
+        drop_graph(c);
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+    rc = ose_run(c, nsteps);
+    if (rc == OSE_OK) {
+        HIPCHK(c, hipEventRecord(e1, c->stream));
+        HIPCHK(c, hipEventSynchronize(e1));
+        float f = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&f, e0, e1));
+        if (ms) *ms = f;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    return collect(c);
+}
+
+int ose_pass_stats(ose_ctx* c, int pass, double* avg_ms, long* launches, double* bytes)
+{
+    if (!c || pass < 0 || pass >= PK_N) return OSE_EINVAL;
+    if (avg_ms) *avg_ms = c->st_n[pass] ? c->st_ms[pass] / c->st_n[pass] : 0.0;
+    if (launches) *launches = c->st_n[pass];
+    if (bytes) *bytes = c->st_bytes[pass];
+    return OSE_OK;
+}
+
+int ose_reset_stats(ose_ctx* c)
+{
+    if (!c) return OSE_EINVAL;
+    int rc = ose_sync(c);
+    for (int i = 0; i < PK_N; ++i) c->st_ms[i] = 0, c->st_n[i] = 0, c->st_bytes[i] = 0;
+    return rc;
+}
+
+int ose_get_step(ose_ctx* c, int* it)
+{
+    if (!c || !it) return OSE_EINVAL;
+    int rc;
+    if ((rc = ose_sync(c))) return rc;
+    HIPCHK(c, hipMemcpy(it, c->d_it, sizeof(int), hipMemcpyDeviceToHost));
+    return OSE_OK;
+}
+
+int ose_set_step(ose_ctx* c, int it)
+{
+    if (!c || it < 0) return OSE_EINVAL;
+    int rc;
+    if ((rc = ose_sync(c))) return rc;
+    HIPCHK(c, hipMemcpy(c->d_it, &it, sizeof(int), hipMemcpyHostToDevice));
+    return OSE_OK;
+}
+
+int ose_get_traces(ose_ctx* c, int shot_id, int it0, int nsteps, float* out)
+{
+    if (!c) return OSE_EINVAL;
+    Shot* sh = get_shot(c, shot_id);
+    if (!sh || !out) return c->fail(OSE_EINVAL, "ose_get_traces: bad shot or null output");
+    int rc;
+    if ((rc = finalize(c))) return rc;
+    Model& m = *c->models[sh->model];
+    if (it0 < 0 || nsteps < 0 || it0 + nsteps > m.nt_cap)
+        return c->fail(OSE_EINVAL, "ose_get_traces: steps [%d,%d) outside the trace buffer (%d)", it0,
+                       it0 + nsteps, m.nt_cap);
+    if ((rc = ose_sync(c))) return rc;
+    const long nrec = (long)m.rec_ny * m.rec_nx;
+    std::vector<float> buf((size_t)nrec * nsteps);
+    const float* src = m.d_traces + (long)sh->slot * m.nt_cap * nrec + (long)it0 * nrec;
+    if (nsteps) HIPCHK(c, hipMemcpy(buf.data(), src, sizeof(float) * buf.size(), hipMemcpyDeviceToHost));
+    for (int it = 0; it < nsteps; ++it)
+        for (long r = 0; r < nrec; ++r) out[r * nsteps + it] = buf[(size_t)it * nrec + r];
+    return OSE_OK;
+}
+
+int ose_get_wavefield(ose_ctx* c, int shot_id, float* re, float* im)
+{
+    if (!c) return OSE_EINVAL;
+    Shot* sh = get_shot(c, shot_id);
+    if (!sh || !re || !im) return c->fail(OSE_EINVAL, "ose_get_wavefield: bad shot or null output");
+    int rc;
+    if ((rc = set_device(c)) || (rc = finalize(c)) || (rc = ose_sync(c))) return rc;
+    Model& m = *c->models[sh->model];
+    XArgs a = base_xargs(m, 1);
+    a.Xprev = m.X[0] + (long)sh->slot * m.ncell;
+    a.A = m.A + (long)sh->slot * m.ncell;
+    const bool prof = c->profile;
+    c->profile = false;
+    rc = launch_x(c, 0, m.ex, XM_INV, a, m.d.nyb * m.d.nzb, 1, 0.0);
+    c->profile = prof;
+    if (rc) return rc;
+    std::vector<float2> buf((size_t)m.ncell);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(buf.data(), a.A, sizeof(float2) * buf.size(), hipMemcpyDeviceToHost));
+    for (long i = 0; i < m.ncell; ++i) {
+        re[i] = buf[i].x;
+        im[i] = buf[i].y;
+    }
+    return OSE_OK;
+}
+
+int ose_set_wavefield(ose_ctx* c, int shot_id, const float* re, const float* im)
+{
+    if (!c) return OSE_EINVAL;
+    Shot* sh = get_shot(c, shot_id);
+    if (!sh || !re || !im) return c->fail(OSE_EINVAL, "ose_set_wavefield: bad shot or null input");
+    int rc;
+    if ((rc = set_device(c)) || (rc = finalize(c)) || (rc = ose_sync(c))) return rc;
+    Model& m = *c->models[sh->model];
+    std::vector<float2> buf((size_t)m.ncell);
+    for (long i = 0; i < m.ncell; ++i) {
+        if (!std::isfinite(re[i]) || !std::isfinite(im[i]))
+            return c->fail(OSE_ENUMERIC, "ose_set_wavefield: non-finite value at cell %ld", i);
+        buf[i] = make_float2(re[i], im[i]);
+    }
+    XArgs a = base_xargs(m, 1);
+    a.A = m.A + (long)sh->slot * m.ncell;
+    a.Xout = m.X[0] + (long)sh->slot * m.ncell;
+    HIPCHK(c, hipMemcpy(a.A, buf.data(), sizeof(float2) * buf.size(), hipMemcpyHostToDevice));
+    const bool prof = c->profile;
+    c->profile = false;
+    rc = launch_x(c, 0, m.ex, XM_FWD, a, m.d.nyb * m.d.nzb, 1, 0.0);
+    c->profile = prof;
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return OSE_OK;
+}
+
+int ose_apply_operator(ose_ctx* c, int model_id, const float* in, float* out)
+{
+    if (!c) return OSE_EINVAL;
+    if (model_id < 0 || model_id >= (int)c->models.size() || !in || !out)
+        return c->fail(OSE_EINVAL, "ose_apply_operator: bad model or null pointer");
+    int rc;
+    if ((rc = set_device(c)) || (rc = ose_sync(c))) return rc;
+    Model& m = *c->models[model_id];
+    const int nx = m.d.nxb, ny = m.d.nyb, nz = m.d.nzb;
+    DevBuf tmp;
+    const size_t fb = sizeof(float2) * (size_t)m.ncell;
+    if ((rc = dev_alloc(c, tmp, 2 * fb))) return rc;
+    float2* A = (float2*)tmp.p;
+    float2* X = (float2*)((char*)tmp.p + fb);
+    HIPCHK(c, hipMemcpy(A, in, fb, hipMemcpyHostToDevice));
+    const bool prof = c->profile;
+    c->profile = false;
+    XArgs a = base_xargs(m, 1);
+    a.A = A;
+    a.Xout = X;
+    rc = launch_x(c, 0, m.ex, XM_FWD, a, ny * nz, 1, 0.0);
+    if (!rc) rc = launch_strided(c, 0, m.ey, m.ey->ystr[0], X, X, nx, ny, nz, false, 1, m.ncell, m.d_twy, nullptr, 0);
+    if (!rc) rc = launch_strided(c, 0, m.ez, m.ez->zsym, X, X, nx, ny, nz, true, 1, m.ncell, m.d_twz, &m, 0);
+    if (!rc) rc = launch_strided(c, 0, m.ey, m.ey->ystr[1], X, X, nx, ny, nz, false, 1, m.ncell, m.d_twy, nullptr, 0);
+    if (!rc) {
+        XArgs b = base_xargs(m, 1);
+        b.Xprev = X;
+        b.A = A;
+        rc = launch_x(c, 0, m.ex, XM_INV, b, ny * nz, 1, 0.0);
+    }
+    if (!rc) {
+        void* args[] = {&A, &m.d_vr, &m.ncell, &m.ncell, nullptr};
+        float s = (float)nx;   // XM_INV divided by nx; the operator's 1/N sits in the symbol
+        args[4] = &s;
+        hipError_t e = hipLaunchKernel((const void*)&k_scale_vr, dim3((unsigned)((m.ncell + 255) / 256), 1),
+                                       dim3(256), args, 0, c->stream);
+        if (e != hipSuccess) rc = c->fail(OSE_EDEVICE, "k_scale_vr launch: %s", hipGetErrorString(e));
+    }
+    c->profile = prof;
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out, A, fb, hipMemcpyDeviceToHost));
+    return OSE_OK;
+}
+
+int ose_fft3d(ose_ctx* c, int nx, int ny, int nz, const float* in, float* out, int sign)
+{
+    if (!c) return OSE_EINVAL;
+    if (!in || !out || (sign != 1 && sign != -1)) return c->fail(OSE_EINVAL, "ose_fft3d: bad argument");
+    const FftEntry *ex = find_entry(nx), *ey = find_entry(ny), *ez = find_entry(nz);
+    if (!ex || !ey || !ez) return c->fail(OSE_EUNSUPPORTED, "ose_fft3d: extent not compiled");
+    int rc;
+    if ((rc = set_device(c)) || (rc = ose_sync(c))) return rc;
+    for (const FftEntry* e : {ex, ey, ez})
+        if ((rc = prepare_kernels(c, e))) return rc;
+    const long n = (long)nx * ny * nz;
+    const size_t fb = sizeof(float2) * (size_t)n;
+    std::vector<float2> tx, ty, tz;
+    roots(nx, tx);
+    roots(ny, ty);
+    roots(nz, tz);
+    DevBuf tmp;
+    if ((rc = dev_alloc(c, tmp, 2 * fb + sizeof(float2) * (nx + ny + nz) + sizeof(float) * (nx + ny + nz))))
+        return rc;
+    float2* A = (float2*)tmp.p;
+    float2* X = A + n;
+    float2* twx = X + n;
+    float2* twy = twx + nx;
+    float2* twz = twy + ny;
+    float* ones = (float*)(twz + nz);
+    std::vector<float> one(nx + ny + nz, 1.f);
+    HIPCHK(c, hipMemcpy(twx, tx.data(), sizeof(float2) * nx, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(twy, ty.data(), sizeof(float2) * ny, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(twz, tz.data(), sizeof(float2) * nz, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(ones, one.data(), sizeof(float) * one.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(A, in, fb, hipMemcpyHostToDevice));
+    const bool prof = c->profile;
+    c->profile = false;
+    XArgs a{};
+    a.tw = twx;
+    a.shot_stride = n;
+    a.ny = ny;
+    a.inv_nx = 1.0f;
+    a.gx = a.gy = a.gz = ones;
+    if (sign < 0) {
+        a.A = A;
+        a.Xout = X;
+        rc = launch_x(c, 0, ex, XM_FWD, a, ny * nz, 1, 0.0);
+        if (!rc) rc = launch_strided(c, 0, ey, ey->ystr[0], X, X, nx, ny, nz, false, 1, n, twy, nullptr, 0);
+        if (!rc) rc = launch_strided(c, 0, ez, ez->ystr[0], X, X, nx, ny, nz, true, 1, n, twz, nullptr, 0);
+    } else {
+        rc = launch_strided(c, 0, ez, ez->ystr[1], A, A, nx, ny, nz, true, 1, n, twz, nullptr, 0);
+        if (!rc) rc = launch_strided(c, 0, ey, ey->ystr[1], A, A, nx, ny, nz, false, 1, n, twy, nullptr, 0);
+        a.Xprev = A;
+        a.A = X;   // XM_INV writes IFFT_x * inv_nx (= 1 here) into A
+        if (!rc) rc = launch_x(c, 0, ex, XM_INV, a, ny * nz, 1, 0.0);
+    }
+    c->profile = prof;
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(out, X, fb, hipMemcpyDeviceToHost));
+    return OSE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,366 @@
+// ose_kernels.h -- CDNA4 (gfx950) kernels of the OSE time step.
+//
+// One Chebyshev order of the one-step extrapolation applies
+//     L W = (v/R) * IFFT3( |k| FFT3(W) ) / N
+// to the packed complex field W_j = (-i)^j Z_j (Re = reference p-lane,
+// Im = q-lane, ose_3d_lib.cu:639-874) and runs the three-term recursion
+//     W_{j+1} = W_{j-1} - 2i L W_j,   A = g (A + c_{j+1} W_{j+1}).
+// It is four HBM passes over complex64 fields laid out [shot][z][y][x]:
+//   1. k_strided<SIGN=-1>  forward-y       X_j  -> S
+//   2. k_zsym              forward-z, * |k|/N, inverse-z   (in place on S)
+//   3. k_strided<SIGN=+1>  inverse-y       (in place on S)
+//   4. k_xpass<MODE>       inverse-x of S and of X_{j-1}, recursion, taper,
+//                          accumulator, receivers, forward-x of W_{j+1}
+// X_j is kept in x-spectral form (FFT_x W_j) so pass 4 recovers W_{j-1}
+// from X_{j-1} with an inverse x transform instead of a second stored field.
+//
+// Every 1D transform of length N = N1*N2 is done in registers and LDS:
+// step A: N2 threads each run an N1-point register DFT over x[N2*n1 + t],
+// multiply by exp(s 2 pi i t k1/N) and write LDS; step B: N1 threads each
+// gather over n2 and run an N2-point register DFT, giving X[k1 + N1*k2].
+// One LDS exchange per transform; global loads and stores are coalesced
+// along x in every pass.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "dft_gen.h"
+
+namespace ose {
+
+template <int A, int B> struct Max { static constexpr int value = A > B ? A : B; };
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+
+// ---------------------------------------------------------------------
+// Strided passes (y and z): a workgroup owns T consecutive x columns of
+// one (z or y) plane and transforms all T pencils of length N.
+// ---------------------------------------------------------------------
+struct StridedArgs {
+    const float2* in;
+    float2* out;
+    int ncol;           // columns along x (nx)
+    long es;            // element stride along the pencil
+    long os;            // stride of the blockIdx.y index
+    long shot_stride;   // cells per shot
+    const float2* tw;   // tw[j] = exp(-2 pi i j / N), j < N
+    // symbol tables (z pass only): squared folded wavenumbers
+    const float* kx2;
+    const float* ky2;
+    const float* kz2;
+    float invN;         // 1 / (nx ny nz)
+};
+
+template <int N1, int N2, int T>
+struct StridedCfg {
+    static constexpr int N = N1 * N2;
+    static constexpr int NT = Max<N1, N2>::value;
+    static constexpr int THREADS = T * NT;
+    static constexpr int LDT = T + 1;                       // padded row
+    static constexpr size_t LDS_BYTES = sizeof(float2) * (size_t)N * LDT;
+};
+
+template <int N1, int N2, int T, int SIGN>
+__global__ __launch_bounds__((StridedCfg<N1, N2, T>::THREADS))
+void k_strided(StridedArgs a)
+{
+    using C = StridedCfg<N1, N2, T>;
+    extern __shared__ float2 lds[];
+    const int tid = threadIdx.x;
+    const int col = tid % T;
+    const int t = tid / T;
+    const int x = blockIdx.x * T + col;
+    const bool ok = x < a.ncol;
+    const long base = (long)blockIdx.z * a.shot_stride + (long)blockIdx.y * a.os + x;
+    float2 v[C::NT];
+    if (t < N2) {
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1)
+            v[n1] = ok ? a.in[base + (long)(N2 * n1 + t) * a.es] : make_float2(0.f, 0.f);
+        rdft<N1, SIGN>(v);
+#pragma unroll
+        for (int k1 = 1; k1 < N1; ++k1) {
+            const float2 w = a.tw[t * k1];
+            v[k1] = SIGN < 0 ? cmul(v[k1], w) : cmulc(v[k1], w);
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) lds[(t * N1 + k1) * C::LDT + col] = v[k1];
+    }
+    __syncthreads();
+    if (t < N1) {
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) v[n2] = lds[(n2 * N1 + t) * C::LDT + col];
+        rdft<N2, SIGN>(v);
+        if (ok) {
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) a.out[base + (long)(t + N1 * k2) * a.es] = v[k2];
+        }
+    }
+}
+
+// z pass: forward-z, multiply by |k|/N (folded wavenumbers, scale_kxkz_3D
+// ose_3d_kernel.cu:101-126 and the 1/N of scal_vr_3d :355), inverse-z.
+// In place; blockIdx.y is the (spectral) y index.
+template <int N1, int N2, int T>
+__global__ __launch_bounds__((StridedCfg<N1, N2, T>::THREADS))
+void k_zsym(StridedArgs a)
+{
+    using C = StridedCfg<N1, N2, T>;
+    extern __shared__ float2 lds[];
+    const int tid = threadIdx.x;
+    const int col = tid % T;
+    const int t = tid / T;
+    const int x = blockIdx.x * T + col;
+    const bool ok = x < a.ncol;
+    const long base = (long)blockIdx.z * a.shot_stride + (long)blockIdx.y * a.os + x;
+    float2 v[C::NT];
+    if (t < N2) {
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1)
+            v[n1] = ok ? a.in[base + (long)(N2 * n1 + t) * a.es] : make_float2(0.f, 0.f);
+        rdft<N1, -1>(v);
+#pragma unroll
+        for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], a.tw[t * k1]);
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) lds[(t * N1 + k1) * C::LDT + col] = v[k1];
+    }
+    __syncthreads();
+    if (t < N1) {
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) v[n2] = lds[(n2 * N1 + t) * C::LDT + col];
+        rdft<N2, -1>(v);
+        const float kxy = (ok ? a.kx2[x] : 0.f) + a.ky2[blockIdx.y];
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) {
+            const float s = sqrtf(kxy + a.kz2[t + N1 * k2]) * a.invN;
+            v[k2].x *= s;
+            v[k2].y *= s;
+        }
+        // inverse: thread m1 = t holds X[m1 + N1 m2]
+        rdft<N2, 1>(v);
+#pragma unroll
+        for (int n2 = 1; n2 < N2; ++n2) v[n2] = cmulc(v[n2], a.tw[t * n2]);
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) lds[(n2 * N1 + t) * C::LDT + col] = v[n2];
+    }
+    __syncthreads();
+    if (t < N2) {
+#pragma unroll
+        for (int m1 = 0; m1 < N1; ++m1) v[m1] = lds[(t * N1 + m1) * C::LDT + col];
+        rdft<N1, 1>(v);
+        if (ok) {
+#pragma unroll
+            for (int n1 = 0; n1 < N1; ++n1) a.out[base + (long)(N2 * n1 + t) * a.es] = v[n1];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------
+// x pass: a workgroup owns P consecutive x lines of one shot.
+// ---------------------------------------------------------------------
+enum XMode : int {
+    XM_ORDER0 = 0,   // W1 = -i L W0;  A = c0 W0 + c1 W1;  out X1
+    XM_MID = 1,      // W' = W_{j-1} - 2i L W_j;  A = g (A + c W');  out X'
+    XM_LAST = 2,     // as MID, u = g (A + c W') is the new state: receivers, out X0
+    XM_FWD = 3,      // out = FFT_x(A)              (state upload)
+    XM_INV = 4,      // A = IFFT_x(Xprev) / nx      (state download)
+};
+
+struct XArgs {
+    const float2* S;        // spectrum after passes 1-3 (natural order along x)
+    const float2* Xprev;    // x-spectral W_{j-1} (W_0 in ORDER0)
+    float2* Xout;           // x-spectral W_{j+1} (X_0 of the next step in LAST)
+    float2* A;              // accumulator / physical state
+    const float* vr;        // v / R, [z][y][x], shared by the shots of a model
+    const float* gx;
+    const float* gy;
+    const float* gz;
+    const float2* tw;       // x root table
+    long shot_stride;
+    int ny;
+    int nlines;             // ny * nz
+    float c0;               // ORDER0: J0
+    float c1;               // ORDER0: 2 J1; MID/LAST: 2 J_{j+1}
+    float inv_nx;
+    // receivers (LAST): Im(u) on the plane z = rec_z over the interior
+    float* traces;          // [shot][nt_cap][rec_ny][rec_nx]
+    long trace_shot_stride;
+    int rec_z, rec_y0, rec_ny, rec_x0, rec_nx;
+    const int* it;          // device step counter
+    int nt_cap;
+};
+
+template <int N1, int N2, int P>
+struct XCfg {
+    static constexpr int N = N1 * N2;
+    static constexpr int NT = Max<N1, N2>::value;
+    static constexpr int THREADS = P * NT;
+    static constexpr int PN = N1 | 1;                 // odd LDS pitch
+    static constexpr int LINE = N2 * PN;
+    static constexpr size_t LDS_BYTES = sizeof(float2) * (size_t)P * LINE;
+};
+
+// inverse x transform of one line: thread t < N1 loads X[t + N1 m2]; on
+// return thread t < N2 holds x[N2 n1 + t] (unnormalised) in res[n1].
+template <int N1, int N2, int PN>
+__device__ __forceinline__ void x_inverse(const float2* __restrict__ src, bool okl, int t,
+                                          float2* __restrict__ L, const float2* __restrict__ tw,
+                                          float2* __restrict__ w, float2* __restrict__ res)
+{
+    if (t < N1) {
+#pragma unroll
+        for (int m2 = 0; m2 < N2; ++m2) w[m2] = okl ? src[t + N1 * m2] : make_float2(0.f, 0.f);
+        rdft<N2, 1>(w);
+#pragma unroll
+        for (int n2 = 1; n2 < N2; ++n2) w[n2] = cmulc(w[n2], tw[t * n2]);
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) L[n2 * PN + t] = w[n2];
+    }
+    __syncthreads();
+    if (t < N2) {
+#pragma unroll
+        for (int m1 = 0; m1 < N1; ++m1) res[m1] = L[t * PN + m1];
+        rdft<N1, 1>(res);
+    }
+    __syncthreads();
+}
+
+// forward x transform: thread t < N2 holds x[N2 n1 + t] in u[n1]; writes
+// X[k] to dst (natural order).
+template <int N1, int N2, int PN>
+__device__ __forceinline__ void x_forward(float2* __restrict__ u, bool okl, int t,
+                                          float2* __restrict__ L, const float2* __restrict__ tw,
+                                          float2* __restrict__ w, float2* dst)
+{
+    if (t < N2) {
+        rdft<N1, -1>(u);
+#pragma unroll
+        for (int k1 = 1; k1 < N1; ++k1) u[k1] = cmul(u[k1], tw[t * k1]);
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1) L[t * PN + k1] = u[k1];
+    }
+    __syncthreads();
+    if (t < N1) {
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) w[n2] = L[n2 * PN + t];
+        rdft<N2, -1>(w);
+        if (okl) {
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) dst[t + N1 * k2] = w[k2];
+        }
+    }
+}
+
+template <int N1, int N2, int P, int MODE>
+__global__ __launch_bounds__((XCfg<N1, N2, P>::THREADS))
+void k_xpass(XArgs a)
+{
+    using C = XCfg<N1, N2, P>;
+    constexpr int N = C::N;
+    extern __shared__ float2 lds[];
+    const int l = threadIdx.x / C::NT;
+    const int t = threadIdx.x % C::NT;
+    const long line = (long)blockIdx.x * P + l;
+    const bool okl = line < a.nlines;
+    const int y = okl ? (int)(line % a.ny) : 0;
+    const int z = okl ? (int)(line / a.ny) : 0;
+    const long lbase = (long)blockIdx.y * a.shot_stride + line * N;
+    const long vbase = line * N;
+    float2* L = lds + l * C::LINE;
+    float2 u[N1];
+    float2 w[C::NT];
+
+    if (MODE == XM_FWD) {
+        if (t < N2 && okl) {
+#pragma unroll
+            for (int n1 = 0; n1 < N1; ++n1) u[n1] = a.A[lbase + N2 * n1 + t];
+        }
+        x_forward<N1, N2, C::PN>(u, okl, t, L, a.tw, w, a.Xout + lbase);
+        return;
+    }
+    if (MODE == XM_INV) {
+        x_inverse<N1, N2, C::PN>(a.Xprev + lbase, okl, t, L, a.tw, w, u);
+        if (t < N2 && okl) {
+#pragma unroll
+            for (int n1 = 0; n1 < N1; ++n1)
+                a.A[lbase + N2 * n1 + t] = make_float2(u[n1].x * a.inv_nx, u[n1].y * a.inv_nx);
+        }
+        return;
+    }
+
+    // L W_j in physical x (unnormalised inverse; the symbol carried 1/N)
+    x_inverse<N1, N2, C::PN>(a.S + lbase, okl, t, L, a.tw, w, u);
+    float2 p[N1];
+    x_inverse<N1, N2, C::PN>(a.Xprev + lbase, okl, t, L, a.tw, w, p);
+
+    if (t < N2 && okl) {
+        const float gyz = a.gy[y] * a.gz[z];
+        int itv = 0;
+        bool rec = false;
+        if (MODE == XM_LAST) {
+            itv = *a.it;
+            rec = (z == a.rec_z) && (y >= a.rec_y0) && (y < a.rec_y0 + a.rec_ny) && (itv < a.nt_cap);
+        }
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) {
+            const int x = N2 * n1 + t;
+            const float vr = a.vr[vbase + x];
+            const float2 lw = make_float2(vr * u[n1].x, vr * u[n1].y);
+            const float2 wp = make_float2(p[n1].x * a.inv_nx, p[n1].y * a.inv_nx);
+            float2 wn;
+            if (MODE == XM_ORDER0) {
+                // W1 = -i L W0
+                wn = make_float2(lw.y, -lw.x);
+                a.A[lbase + x] = make_float2(a.c0 * wp.x + a.c1 * wn.x, a.c0 * wp.y + a.c1 * wn.y);
+            } else {
+                // W_{j+1} = W_{j-1} - 2i L W_j
+                wn = make_float2(wp.x + 2.f * lw.y, wp.y - 2.f * lw.x);
+                const float g = gyz * a.gx[x];
+                const float2 acc = a.A[lbase + x];
+                const float2 an = make_float2(g * (acc.x + a.c1 * wn.x), g * (acc.y + a.c1 * wn.y));
+                if (MODE == XM_MID) {
+                    a.A[lbase + x] = an;
+                } else {
+                    wn = an;   // the new state u; its x transform is X_0 of the next step
+                    if (rec && x >= a.rec_x0 && x < a.rec_x0 + a.rec_nx)
+                        a.traces[(long)blockIdx.y * a.trace_shot_stride +
+                                 ((long)itv * a.rec_ny + (y - a.rec_y0)) * a.rec_nx + (x - a.rec_x0)] = an.y;
+                }
+            }
+            u[n1] = wn;
+        }
+    }
+    x_forward<N1, N2, C::PN>(u, okl, t, L, a.tw, w, a.Xout + lbase);
+}
+
+// ---------------------------------------------------------------------
+// Source injection in x-spectral form and the step counter.
+// u[src] += s[it] + i h[it] before each step (ose_3d.cu:252-253,
+// add_wavelet_Xt ose_3d_kernel.cu:291-297) becomes, on X_0 = FFT_x(u),
+// X_0[line][k] += (s + i h) exp(-2 pi i k sx / nx).
+// ---------------------------------------------------------------------
+struct InjectArgs {
+    float2* X0;
+    long shot_stride;
+    int nx;
+    const long* src_line;   // per shot: (sz*ny + sy) * nx
+    const int* src_x;       // per shot
+    const float* s;         // [shot][nt]
+    const float* h;
+    int nt;
+    const int* it;
+    const float2* tw;       // x root table
+};
+
+__global__ void k_inject(InjectArgs a);
+__global__ void k_advance(int* it);
+__global__ void k_scale_vr(float2* f, const float* vr, long n, long shot_stride, float s);
+
+}  // namespace ose

@@ -1,0 +1,101 @@
+"""ctypes loading of the in-tree native libraries.
+
+libose_hip.so is the only way to the GPU: if it is missing or fails to load,
+every call raises.  There is no CPU fallback.
+"""
+import ctypes
+import os
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_DIR = os.path.join(PKG_ROOT, "lib")
+
+_hip = None
+_host = None
+
+c_int, c_long, c_double, c_float = ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_float
+P = ctypes.POINTER
+fp, dp, ip = P(c_float), P(c_double), P(c_int)
+
+
+class OseError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"OSE error {code}: {msg}")
+        self.code = code
+
+
+OSE_OK, OSE_EINVAL, OSE_ENOMEM, OSE_EDEVICE, OSE_ESTATE, OSE_ENUMERIC, OSE_EUNSUPPORTED = range(7)
+
+
+class ModelDesc(ctypes.Structure):
+    _fields_ = [
+        ("nxb", c_int), ("nyb", c_int), ("nzb", c_int),
+        ("bl", c_int), ("bf", c_int), ("bu", c_int),
+        ("nx", c_int), ("ny", c_int), ("nz", c_int),
+        ("dx", c_double), ("dy", c_double), ("dz", c_double), ("dt", c_double),
+        ("R", c_double), ("K", c_int), ("coef", dp),
+        ("taper_x", fp), ("taper_y", fp), ("taper_z", fp),
+        ("rec_z", c_int),
+    ]
+
+
+def _proto(lib, name, args, res=c_int):
+    f = getattr(lib, name)
+    f.argtypes = args
+    f.restype = res
+
+
+def hip():
+    """Load libose_hip.so (the HIP backend)."""
+    global _hip
+    if _hip is None:
+        path = os.path.join(LIB_DIR, "libose_hip.so")
+        if not os.path.exists(path):
+            raise OseError(OSE_EDEVICE, f"{path} is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(path)
+        vp = ctypes.c_void_p
+        _proto(L, "ose_create", [P(vp), c_int])
+        _proto(L, "ose_destroy", [vp])
+        _proto(L, "ose_last_error", [vp], ctypes.c_char_p)
+        _proto(L, "ose_add_model", [vp, P(ModelDesc), fp, ip])
+        _proto(L, "ose_add_shot", [vp, c_int, c_int, c_int, c_int, fp, fp, c_int, ip])
+        _proto(L, "ose_run", [vp, c_int])
+        _proto(L, "ose_sync", [vp])
+        _proto(L, "ose_timed_run", [vp, c_int, dp])
+        _proto(L, "ose_get_traces", [vp, c_int, c_int, c_int, fp])
+        _proto(L, "ose_get_wavefield", [vp, c_int, fp, fp])
+        _proto(L, "ose_set_wavefield", [vp, c_int, fp, fp])
+        _proto(L, "ose_get_step", [vp, ip])
+        _proto(L, "ose_set_step", [vp, c_int])
+        _proto(L, "ose_apply_operator", [vp, c_int, fp, fp])
+        _proto(L, "ose_fft3d", [vp, c_int, c_int, c_int, fp, fp, c_int])
+        _proto(L, "ose_set_options", [vp, c_int, c_int])
+        _proto(L, "ose_pass_stats", [vp, c_int, dp, P(c_long), dp])
+        _proto(L, "ose_reset_stats", [vp])
+        _proto(L, "ose_supported_extent", [c_int])
+        _proto(L, "ose_extent_is_supported", [c_int])
+        _hip = L
+    return _hip
+
+
+def host():
+    """Load libose_host.so (C++ host library: wavelets, Bessel, models, I/O)."""
+    global _host
+    if _host is None:
+        path = os.path.join(LIB_DIR, "libose_host.so")
+        if not os.path.exists(path):
+            raise OseError(OSE_EINVAL, f"{path} is not built (run __graft_entry__.build())")
+        _host = ctypes.CDLL(path)
+        from . import _host_protos
+        _host_protos.declare(_host)
+    return _host
+
+
+def hip_symbols():
+    """Names declared in include/ose_api.h (checked against the .so exports)."""
+    return [
+        "ose_create", "ose_destroy", "ose_last_error", "ose_add_model", "ose_add_shot",
+        "ose_run", "ose_sync", "ose_timed_run", "ose_get_traces", "ose_get_wavefield",
+        "ose_set_wavefield", "ose_get_step", "ose_set_step", "ose_apply_operator",
+        "ose_fft3d", "ose_set_options", "ose_pass_stats", "ose_reset_stats",
+        "ose_supported_extent", "ose_extent_is_supported",
+    ]

@@ -1,0 +1,171 @@
+"""Python view of the HIP backend's C-ABI (include/ose_api.h).
+
+Context mirrors the reference's device-side objects: a model is the padded
+velocity plus the step parameters wavefield_extro_ori2 reads from globals
+(ose_3d.cu:42-62), a shot is a source position with its two wavelets
+(Xt_add_wavelet_sxyz, ose_3d_lib.cu:610-633), run() is the time loop body
+(ose_3d.cu:248-275) and traces() the seismogram (ose_3d.cu:271-276).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import ModelDesc, OseError, c_double, c_int, c_long
+
+_fp = ctypes.POINTER(ctypes.c_float)
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_fp)
+
+
+class Context:
+    def __init__(self, device=0):
+        self._L = _lib.hip()
+        h = ctypes.c_void_p()
+        rc = self._L.ose_create(ctypes.byref(h), device)
+        if rc:
+            raise OseError(rc, f"ose_create(device={device}) failed")
+        self._h = h
+        self._keep = []
+        self.models = []
+        self.shots = []
+
+    # -- lifetime -----------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.ose_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc:
+            msg = self._L.ose_last_error(self._h)
+            raise OseError(rc, msg.decode() if msg else "")
+
+    # -- set-up -------------------------------------------------------
+    def add_model(self, v_padded, *, interior, origin, d, dt, R, coef, taper, rec_z=0):
+        """v_padded: [nzb][nyb][nxb] velocity; interior = (nx, ny, nz);
+        origin = (bl, bf, bu); d = (dx, dy, dz); coef: K+1 doubles;
+        taper = (gx, gy, gz)."""
+        v = _f32(v_padded)
+        nzb, nyb, nxb = v.shape
+        coef = np.ascontiguousarray(coef, dtype=np.float64)
+        gx, gy, gz = (_f32(t) for t in taper)
+        desc = ModelDesc(nxb, nyb, nzb, origin[0], origin[1], origin[2],
+                         interior[0], interior[1], interior[2], d[0], d[1], d[2], dt,
+                         float(R), len(coef) - 1, coef.ctypes.data_as(_dp),
+                         _ptr(gx), _ptr(gy), _ptr(gz), int(rec_z))
+        mid = c_int()
+        self._check(self._L.ose_add_model(self._h, ctypes.byref(desc), _ptr(v), ctypes.byref(mid)))
+        self.models.append(dict(shape=v.shape, interior=tuple(interior), origin=tuple(origin)))
+        return mid.value
+
+    def add_shot(self, model, src, s, h):
+        """src: padded (sx, sy, sz); s, h: wavelets added to Re u and Im u."""
+        s, h = _f32(s), _f32(h)
+        assert s.size == h.size
+        sid = c_int()
+        self._check(self._L.ose_add_shot(self._h, model, int(src[0]), int(src[1]), int(src[2]),
+                                         _ptr(s), _ptr(h), s.size, ctypes.byref(sid)))
+        self.shots.append(dict(model=model, nt=s.size))
+        return sid.value
+
+    def set_options(self, use_graph=True, profile=False):
+        self._check(self._L.ose_set_options(self._h, int(use_graph), int(profile)))
+
+    # -- stepping -----------------------------------------------------
+    def run(self, nsteps):
+        self._check(self._L.ose_run(self._h, int(nsteps)))
+
+    def sync(self):
+        self._check(self._L.ose_sync(self._h))
+
+    def timed_run(self, nsteps):
+        ms = c_double()
+        self._check(self._L.ose_timed_run(self._h, int(nsteps), ctypes.byref(ms)))
+        return ms.value
+
+    @property
+    def step(self):
+        it = c_int()
+        self._check(self._L.ose_get_step(self._h, ctypes.byref(it)))
+        return it.value
+
+    @step.setter
+    def step(self, it):
+        self._check(self._L.ose_set_step(self._h, int(it)))
+
+    # -- results ------------------------------------------------------
+    def traces(self, shot, it0=0, nsteps=None):
+        m = self.models[self.shots[shot]["model"]]
+        nx, ny, _ = m["interior"]
+        if nsteps is None:
+            nsteps = self.shots[shot]["nt"] - it0
+        out = np.empty((nx * ny, nsteps), dtype=np.float32)
+        self._check(self._L.ose_get_traces(self._h, shot, it0, nsteps, _ptr(out)))
+        return out
+
+    def wavefield(self, shot):
+        shape = self.models[self.shots[shot]["model"]]["shape"]
+        re = np.empty(shape, dtype=np.float32)
+        im = np.empty(shape, dtype=np.float32)
+        self._check(self._L.ose_get_wavefield(self._h, shot, _ptr(re), _ptr(im)))
+        return re, im
+
+    def set_wavefield(self, shot, re, im):
+        re, im = _f32(re), _f32(im)
+        self._check(self._L.ose_set_wavefield(self._h, shot, _ptr(re), _ptr(im)))
+
+    def apply_operator(self, model, u):
+        u = np.ascontiguousarray(u, dtype=np.complex64)
+        out = np.empty_like(u)
+        self._check(self._L.ose_apply_operator(self._h, model, u.view(np.float32).ctypes.data_as(_fp),
+                                               out.view(np.float32).ctypes.data_as(_fp)))
+        return out
+
+    def fft3d(self, a, sign=-1):
+        a = np.ascontiguousarray(a, dtype=np.complex64)
+        out = np.empty_like(a)
+        nz, ny, nx = a.shape
+        self._check(self._L.ose_fft3d(self._h, nx, ny, nz, a.view(np.float32).ctypes.data_as(_fp),
+                                      out.view(np.float32).ctypes.data_as(_fp), sign))
+        return out
+
+    def pass_stats(self):
+        """{pass: (avg_ms, launches, bytes_per_launch)} for the 7 pass kinds."""
+        out = {}
+        names = ["fwd_y", "z_symbol", "inv_y", "x_epilogue", "x_first", "x_last", "inject"]
+        for i, n in enumerate(names):
+            ms, cnt, by = c_double(), c_long(), c_double()
+            self._check(self._L.ose_pass_stats(self._h, i, ctypes.byref(ms), ctypes.byref(cnt), ctypes.byref(by)))
+            out[n] = (ms.value, cnt.value, by.value)
+        return out
+
+    def reset_stats(self):
+        self._check(self._L.ose_reset_stats(self._h))
+
+
+def supported_extent(n):
+    return _lib.hip().ose_supported_extent(int(n))
+
+
+def extent_is_supported(n):
+    return bool(_lib.hip().ose_extent_is_supported(int(n)))

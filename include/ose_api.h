@@ -1,0 +1,133 @@
+/*
+ * ose_api.h -- the extern "C" boundary of the MI355X OSE time stepper.
+ *
+ * The reference (Littleke/3D-OSE) has no plugin or operator API: its hot
+ * path is the C function wavefield_extro_ori2(cufftHandle, cufftResult)
+ * (ose_3d_lib.cu:639) reading ~20 program globals (ose_3d.cu:42-62), fed by
+ * Xt_add_wavelet_sxyz (ose_3d_lib.cu:610) and read by rec_shotdata_3d
+ * (ose_3d_kernel.cu:412) + rec_slice_to_data (ose_3d_lib.cu:876).  This
+ * header is what that path's callers bind instead.  Each entry point names
+ * the reference code it replaces.
+ *
+ * Conventions
+ *   - Arrays are [z][y][x] with x fastest (ose_3d_kernel.cu:35); "padded"
+ *     extents include the absorbing pads.
+ *   - Host pointers are caller-owned and only read or written during the
+ *     call; the library owns all device memory.
+ *   - Every function returns an OSE_* status; nothing exits the process.
+ *     ose_last_error() gives the text of the last failure on a context.
+ *   - One host thread per context; ose_run is asynchronous on the
+ *     context's HIP stream, getters synchronize.
+ */
+#ifndef OSE_API_H
+#define OSE_API_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OSE_OK            0
+#define OSE_EINVAL        1   /* bad argument                                  */
+#define OSE_ENOMEM        2   /* device or host allocation failed              */
+#define OSE_EDEVICE       3   /* HIP runtime error                             */
+#define OSE_ESTATE        4   /* call not valid in the context's current state */
+#define OSE_ENUMERIC      5   /* non-finite input                              */
+#define OSE_EUNSUPPORTED  6   /* FFT extent not compiled                       */
+
+typedef struct ose_ctx ose_ctx;
+
+/* One velocity model on one padded grid.  Replaces the globals
+ * nxb,nyb,nzb, dkx..dkz, R, dtR, M and att_blr/att_bfb/att_bud that
+ * wavefield_extro_ori2 reads (ose_3d.cu:42-62, 120-122, 191-199;
+ * Alloc_mulGPU ose_3d_lib.cu:123-159).                                     */
+typedef struct {
+    int nxb, nyb, nzb;     /* padded extents = FFT sizes                       */
+    int bl, bf, bu;        /* interior origin inside the padded grid           */
+    int nx, ny, nz;        /* interior extents                                 */
+    double dx, dy, dz, dt; /* grid spacing (m), time step (s)                  */
+    double R;              /* vmax*pi*sqrt(dx^-2+dy^-2+dz^-2)  (ose_3d.cu:194) */
+    int K;                 /* highest Bessel order = operator applications per
+                              step, >= 2 (reference: M+1, ose_3d.cu:197)       */
+    const double* coef;    /* K+1 values: J0(R dt), 2 J_k(R dt), k = 1..K      */
+    const float* taper_x;  /* nxb / nyb / nzb accumulator taper profiles       */
+    const float* taper_y;  /*   (generate_att, ose_3d_kernel.cu:242-261)       */
+    const float* taper_z;
+    int rec_z;             /* receiver plane: interior depth index (ref: 0)    */
+} ose_model_desc;
+
+/* Context on one HIP device.  Replaces Alloc/Alloc_cufftXt/cufftXtSetGPUs/
+ * cufftMakePlan3d (ose_3d_lib.cu:47-105,162-184; ose_3d.cu:207-237).     */
+int ose_create(ose_ctx** ctx, int device);
+/* Replaces Free_cufftXt/cufftDestroy/Free (ose_3d_lib.cu:186-250).         */
+int ose_destroy(ose_ctx* ctx);
+const char* ose_last_error(const ose_ctx* ctx);
+
+/* Register a model.  v_zyx: padded velocity [nzb][nyb][nxb] in m/s (the
+ * caller edge-pads as add_pml_layers_v_h, ose_3d_lib.cu:912-1032, does).
+ * Replaces the H2D of vbD (ose_3d.cu:236) and the taper set-up.            */
+int ose_add_model(ose_ctx* ctx, const ose_model_desc* desc, const float* v_zyx,
+                  int* model_id);
+
+/* Register a shot on a model: source at padded cell (sx, sy, sz), wavelets
+ * s (added to Re u) and h (added to Im u) of length nt, injected before
+ * each step (Xt_add_wavelet_sxyz ose_3d_lib.cu:610-633, add_wavelet_Xt
+ * ose_3d_kernel.cu:291-297).  Traces are kept for nt steps.                */
+int ose_add_shot(ose_ctx* ctx, int model_id, int sx, int sy, int sz,
+                 const float* s, const float* h, int nt, int* shot_id);
+
+/* Enqueue nsteps OSE time steps for every shot of every model (the body of
+ * the time loop ose_3d.cu:248-275: inject, wavefield_extro_ori2,
+ * rec_shotdata_3d).  Asynchronous; the first call allocates the state.   */
+int ose_run(ose_ctx* ctx, int nsteps);
+int ose_sync(ose_ctx* ctx);
+/* Same as ose_run + ose_sync, timed with HIP events on the context stream. */
+int ose_timed_run(ose_ctx* ctx, int nsteps, double* ms);
+
+/* Receiver traces Im(u) on the plane rec_z over the interior x, y, for steps
+ * [it0, it0+nsteps): out is trace-major [iy*nx+ix][nsteps] like the
+ * reference's seismogram (rec_shotdata_3d ose_3d_kernel.cu:412-436,
+ * rec_slice_to_data ose_3d_lib.cu:876-884, write ose_3d.cu:276).          */
+int ose_get_traces(ose_ctx* ctx, int shot_id, int it0, int nsteps, float* out);
+
+/* Current state u = P + iQ of a shot over the padded grid (the p2D/q2D .y
+ * lanes; snapshot dump ose_3d.cu:260-270).                                 */
+int ose_get_wavefield(ose_ctx* ctx, int shot_id, float* re, float* im);
+/* Overwrite the state of a shot (checkpoint restore; initial conditions). */
+int ose_set_wavefield(ose_ctx* ctx, int shot_id, const float* re, const float* im);
+/* Step counter: index into the wavelets and traces of the next step.       */
+int ose_get_step(ose_ctx* ctx, int* it);
+int ose_set_step(ose_ctx* ctx, int it);
+
+/* out = (v/R) IFFT3(|k| FFT3(in)) / N for one complex field (interleaved
+ * re,im, padded grid of the model): the operator application of
+ * ose_3d_lib.cu:649-687 (memcpy_between_device_cr, cufftXtExec FORWARD,
+ * scale_kxkz_3D, INVERSE, scal_vr_3d).                                      */
+int ose_apply_operator(ose_ctx* ctx, int model_id, const float* in, float* out);
+
+/* Unnormalised 3D DFT of an interleaved complex [nz][ny][nx] array with the
+ * library's pass kernels; sign -1 forward, +1 inverse (cufftXtExecDescriptorC2C,
+ * ose_3d_lib.cu:668).                                                       */
+int ose_fft3d(ose_ctx* ctx, int nx, int ny, int nz, const float* in, float* out, int sign);
+
+/* use_graph: replay one captured hipGraph per step (default 1).
+ * profile:   record HIP events around every pass kernel (eager launches)
+ *            so ose_pass_stats can report per-pass durations.              */
+int ose_set_options(ose_ctx* ctx, int use_graph, int profile);
+/* Per-pass statistics since the last reset: pass 0 forward-y, 1 z-symbol,
+ * 2 inverse-y, 3 x-epilogue (middle orders), 4 x-epilogue (first order),
+ * 5 x-epilogue (last order), 6 injection.  avg_ms is the mean HIP-event
+ * duration of one launch, bytes the algorithmic HBM bytes of one launch.   */
+int ose_pass_stats(ose_ctx* ctx, int pass, double* avg_ms, long* launches, double* bytes);
+int ose_reset_stats(ose_ctx* ctx);
+
+/* Smallest compiled FFT extent >= n, or 0.                                 */
+int ose_supported_extent(int n);
+/* Nonzero if extent n is compiled.                                         */
+int ose_extent_is_supported(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OSE_API_H */
