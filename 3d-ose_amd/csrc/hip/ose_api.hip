@@ -283,6 +283,8 @@ int launch_strided(ose_ctx* c, int kind, const FftEntry* e, const void* fn, cons
                    float2* out, int nx, int ny, int nz, bool zaxis, int nshot, long shot_stride,
                    const float2* tw, const Model* m, double bytes)
 {
+    if (!in || !out || !tw || (fn == e->zsym && !m))
+        return c->fail(OSE_EINVAL, "strided pass launched with a missing buffer");
     StridedArgs a{};
     a.in = in;
     a.out = out;
@@ -305,6 +307,19 @@ int launch_x(ose_ctx* c, int kind, const FftEntry* e, int mode, XArgs& a, int nl
              double bytes)
 {
     a.nlines = nlines;
+    // every pointer the mode dereferences must be set before launching
+    bool ok = a.tw != nullptr;
+    switch (mode) {
+    case XM_FWD: ok = ok && a.A && a.Xout; break;
+    case XM_INV: ok = ok && a.A && a.Xprev; break;
+    case XM_ORDER0: ok = ok && a.S && a.Xprev && a.Xout && a.A && a.vr; break;
+    case XM_MID: ok = ok && a.S && a.Xprev && a.Xout && a.A && a.vr && a.gx && a.gy && a.gz; break;
+    case XM_LAST:
+        ok = ok && a.S && a.Xprev && a.Xout && a.A && a.vr && a.gx && a.gy && a.gz && a.traces && a.it;
+        break;
+    default: ok = false;
+    }
+    if (!ok) return c->fail(OSE_EINVAL, "x pass mode %d launched with a missing buffer", mode);
     dim3 grid((nlines + e->x_P - 1) / e->x_P, nshot);
     return launch(c, kind, bytes, e->xk[mode], grid, dim3(e->x_threads), e->x_lds, &a);
 }
@@ -377,6 +392,7 @@ int enqueue_model_step(ose_ctx* c, Model& m, int* d_it)
             return rc;
         XArgs a = base_xargs(m, nshot);
         a.S = m.S;
+        a.A = m.A;
         int mode, kind;
         double bytes;
         if (j == 0) {

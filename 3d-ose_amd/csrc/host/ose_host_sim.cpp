@@ -1,0 +1,499 @@
+// ose_host_sim.cpp -- the simulation object behind the ose3d driver: the
+// parameter set of the reference's main() (ose_3d.cu:75-199) as SU-style
+// key=value pairs and presets, model and shot construction, and the time
+// loop through include/ose_api.h.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "ose_host.h"
+
+namespace {
+
+struct ShotSpec {
+    int x, y, z;   // interior coordinates
+};
+
+struct ModelSpec {
+    int nx, ny, nz;
+    int kind;          // oseh_model kind, or -1 for a velocity file
+    uint64_t seed;
+    std::vector<ShotSpec> shots;
+};
+
+struct BuiltModel {
+    ModelSpec spec;
+    int bl, br, bf, bb, bu, bd, nxb, nyb, nzb;
+    std::vector<float> v;   // padded
+    double vmax, R, z;
+    int K;
+    std::vector<double> coef;
+    std::vector<float> gx, gy, gz;
+    int id = -1;
+    std::vector<int> shot_ids;
+};
+
+struct ShotRec {
+    int model;
+    int src[3];   // padded
+};
+
+}  // namespace
+
+struct oseh_sim {
+    std::map<std::string, std::string> p;
+    std::vector<ModelSpec> specs;
+    std::vector<BuiltModel> models;
+    std::vector<ShotRec> shots;
+    std::vector<float> s, h;
+    ose_ctx* ctx = nullptr;
+    std::string err;
+    int steps_done = 0;
+
+    int fail(int code, const std::string& m)
+    {
+        err = m;
+        return code;
+    }
+    std::string get(const std::string& k, const std::string& d) const
+    {
+        auto it = p.find(k);
+        return it == p.end() ? d : it->second;
+    }
+    double getd(const std::string& k, double d) const
+    {
+        auto it = p.find(k);
+        return it == p.end() ? d : std::atof(it->second.c_str());
+    }
+    int geti(const std::string& k, int d) const
+    {
+        auto it = p.find(k);
+        return it == p.end() ? d : std::atoi(it->second.c_str());
+    }
+};
+
+namespace {
+
+void parse_into(std::map<std::string, std::string>& p, const std::string& s)
+{
+    std::istringstream is(s);
+    std::string tok;
+    while (is >> tok) {
+        const size_t eq = tok.find('=');
+        if (eq == std::string::npos) continue;
+        p[tok.substr(0, eq)] = tok.substr(eq + 1);
+    }
+}
+
+// Benchmark configurations.  Sizes, velocities, sources and receivers follow
+// the configuration texts; unstated spacing and time step are 10 m and 1 ms.
+const char* preset_text(const std::string& name)
+{
+    if (name == "ref")   // ose_3d.cu:75-119 as shipped
+        return "nx=101 ny=101 nz=101 boundary=25 nt=300 fm=40 dt=0.001 amp=100 dx=10 dy=10 dz=10 "
+               "model=twolayer v0=3000 v1=4000 iface=30 sx=50 sy=50 sz=0 rec_z=0 korder=ref "
+               "taper=ref hilbert=unnorm extent=exact delay=40";
+    if (name == "c0")
+        return "nx=100 ny=100 nz=100 boundary=20 nt=1000 fm=25 dt=0.001 amp=100 dx=10 dy=10 dz=10 "
+               "model=twolayer v0=2500 v1=3500 iface=50 sx=50 sy=50 sz=10 rec_z=5 korder=tail "
+               "ktol=1e-7 taper=cerjan alpha=0.015 hilbert=unnorm extent=exact delay=40";
+    if (name == "c1")
+        return "nx=256 ny=256 nz=256 boundary=40 nt=2000 fm=20 dt=0.001 amp=100 dx=10 dy=10 dz=10 "
+               "model=layered8 seed=2 sx=128 sy=128 sz=0 rec_z=0 korder=tail ktol=1e-7 taper=ref "
+               "hilbert=unnorm extent=smooth delay=40";
+    if (name == "c2")
+        return "batch=c2 boundary=40 nt=1500 fm=20 dt=0.001 amp=100 dx=10 dy=10 dz=10 rec_z=0 "
+               "korder=tail ktol=1e-7 taper=ref hilbert=unnorm extent=smooth delay=40";
+    if (name == "c3")
+        return "nx=256 ny=256 nz=256 boundary=8 nt=2000 fm=20 dt=0.001 amp=100 dx=10 dy=10 dz=10 "
+               "model=layered8 seed=2000 sx=128 sy=128 sz=0 rec_z=0 korder=tail ktol=1e-7 taper=ref "
+               "hilbert=unnorm extent=smooth delay=40";
+    if (name == "c4")
+        return "nx=768 ny=768 nz=768 boundary=40 nt=3000 fm=20 dt=0.001 amp=100 dx=10 dy=10 dz=10 "
+               "model=salt12 seed=5 batch=c4 rec_z=0 korder=tail ktol=1e-7 taper=ref hilbert=unnorm "
+               "extent=smooth delay=40";
+    return nullptr;
+}
+
+int model_kind(const std::string& m)
+{
+    if (m == "const") return 0;
+    if (m == "twolayer") return 1;
+    if (m == "layered8") return 2;
+    if (m == "salt12") return 3;
+    if (m == "train") return 4;
+    if (m == "file") return -1;
+    return -2;
+}
+
+int build_specs(oseh_sim* s)
+{
+    s->specs.clear();
+    const std::string batch = s->get("batch", "");
+    if (batch == "c2") {
+        ModelSpec a{512, 512, 384, 2, 3, {}};
+        for (int iy = 0; iy < 4; ++iy)
+            for (int ix = 0; ix < 4; ++ix) a.shots.push_back({64 + 128 * ix, 64 + 128 * iy, 0});
+        s->specs.push_back(a);
+        const int dims[4][3] = {{192, 192, 192}, {256, 256, 200}, {320, 240, 160}, {288, 160, 224}};
+        for (int k = 0; k < 4; ++k) {
+            ModelSpec m{dims[k][0], dims[k][1], dims[k][2], 2, (uint64_t)(31 + k), {}};
+            m.shots.push_back({dims[k][0] / 2, dims[k][1] / 2, 0});
+            s->specs.push_back(m);
+        }
+    } else {
+        ModelSpec m{s->geti("nx", 64), s->geti("ny", 64), s->geti("nz", 64),
+                    model_kind(s->get("model", "layered8")), (uint64_t)s->geti("seed", 1), {}};
+        if (s->get("vfile", "") != "") m.kind = -1;
+        if (m.kind == -2) return s->fail(OSE_EINVAL, "unknown model " + s->get("model", ""));
+        if (batch == "c4") {
+            for (int iy = 0; iy < 2; ++iy)
+                for (int ix = 0; ix < 4; ++ix)
+                    m.shots.push_back({(int)(m.nx * (0.2 + 0.2 * ix)), (int)(m.ny * (0.35 + 0.3 * iy)), 0});
+        } else if (s->p.count("srcgrid")) {
+            const int g = s->geti("srcgrid", 1);
+            for (int iy = 0; iy < g; ++iy)
+                for (int ix = 0; ix < g; ++ix)
+                    m.shots.push_back({(int)((ix + 0.5) * m.nx / g), (int)((iy + 0.5) * m.ny / g), s->geti("sz", 0)});
+        } else {
+            // shots on a line: (sx + is*jsx, sy + is*jsy, sz + is*jsz), sg_init_host ose_3d_lib.cu:598-608
+            const int ns = s->geti("nshot", 1);
+            for (int k = 0; k < ns; ++k)
+                m.shots.push_back({s->geti("sx", m.nx / 2) + k * s->geti("jsx", 1),
+                                   s->geti("sy", m.ny / 2) + k * s->geti("jsy", 1),
+                                   s->geti("sz", 0) + k * s->geti("jsz", 1)});
+        }
+        s->specs.push_back(m);
+    }
+    const int cap = s->geti("max_shots", 0);
+    if (cap > 0)
+        for (auto& m : s->specs)
+            if ((int)m.shots.size() > cap) m.shots.resize(cap);
+    return OSE_OK;
+}
+
+// pads of one axis: before/after, the odd rule, then the extent rule
+int axis_pads(oseh_sim* s, int n, int b, int& before, int& after, int& nb)
+{
+    before = b;
+    after = b;
+    nb = before + n + after;
+    oseh_judge_odd(&after, &nb);
+    const std::string ext = s->get("extent", "smooth");
+    if (!ose_extent_is_supported(nb)) {
+        if (ext == "exact")
+            return s->fail(OSE_EUNSUPPORTED, "padded extent " + std::to_string(nb) +
+                                                 " is not compiled (use extent=smooth)");
+        const int t = ose_supported_extent(nb);
+        if (t <= 0) return s->fail(OSE_EUNSUPPORTED, "no compiled extent >= " + std::to_string(nb));
+        after += t - nb;   // the taper is extended over the extra cells
+        nb = t;
+    }
+    return OSE_OK;
+}
+
+int build_model(oseh_sim* s, const ModelSpec& spec, BuiltModel& m)
+{
+    m.spec = spec;
+    const int b = s->geti("boundary", 20);
+    int rc;
+    if ((rc = axis_pads(s, spec.nx, s->geti("bl", b), m.bl, m.br, m.nxb))) return rc;
+    if ((rc = axis_pads(s, spec.ny, s->geti("bf", b), m.bf, m.bb, m.nyb))) return rc;
+    if ((rc = axis_pads(s, spec.nz, s->geti("bu", b), m.bu, m.bd, m.nzb))) return rc;
+    std::vector<float> vi((size_t)spec.nx * spec.ny * spec.nz);
+    if (spec.kind == -1) {
+        rc = oseh_read_velocity_xyz(s->get("vfile", "").c_str(), spec.nx, spec.ny, spec.nz, vi.data());
+        if (rc) return s->fail(rc, "cannot read velocity file " + s->get("vfile", ""));
+    } else {
+        rc = oseh_model(spec.kind, spec.nx, spec.ny, spec.nz, spec.seed, s->getd("v0", 2000.0),
+                        s->getd("v1", 3500.0), s->geti("iface", spec.nz / 2), vi.data());
+        if (rc) return s->fail(rc, "model generation failed");
+    }
+    m.v.resize((size_t)m.nxb * m.nyb * m.nzb);
+    oseh_pad_velocity(vi.data(), spec.nx, spec.ny, spec.nz, m.bl, m.br, m.bf, m.bb, m.bu, m.bd, m.v.data());
+    m.vmax = *std::max_element(m.v.begin(), m.v.end());   // find_max, ose_3d_lib.cu:544-555
+    m.R = oseh_spectral_radius(m.vmax, s->getd("dx", 10), s->getd("dy", 10), s->getd("dz", 10));
+    m.z = m.R * s->getd("dt", 1e-3);
+    const std::string ko = s->get("korder", "tail");
+    if (ko == "ref")
+        m.K = oseh_order_reference(m.z);
+    else if (ko == "tail")
+        m.K = oseh_order_tail(m.z, s->getd("ktol", 1e-7));
+    else
+        m.K = std::atoi(ko.c_str());
+    if (m.K < 2 || m.K > 64) return s->fail(OSE_EINVAL, "order K out of range");
+    m.coef.resize(m.K + 1);
+    if ((rc = oseh_coefficients(m.z, m.K, m.coef.data()))) return s->fail(rc, "Bessel coefficients failed");
+    const std::string tp = s->get("taper", "ref");
+    const int mode = tp == "ref" ? OSEH_TAPER_REF : tp == "cerjan" ? OSEH_TAPER_CERJAN : OSEH_TAPER_NONE;
+    const double alpha = s->getd("alpha", 0.015);
+    m.gx.resize(m.nxb);
+    m.gy.resize(m.nyb);
+    m.gz.resize(m.nzb);
+    oseh_taper(m.gx.data(), m.bl, spec.nx, m.br, mode, alpha);
+    oseh_taper(m.gy.data(), m.bf, spec.ny, m.bb, mode, alpha);
+    oseh_taper(m.gz.data(), m.bu, spec.nz, m.bd, mode, alpha);
+    return OSE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oseh_sim_create(oseh_sim** out, const char* params)
+{
+    if (!out) return OSE_EINVAL;
+    *out = nullptr;
+    auto* s = new (std::nothrow) oseh_sim();
+    if (!s) return OSE_ENOMEM;
+    std::map<std::string, std::string> user;
+    parse_into(user, params ? params : "");
+    auto pr = user.find("preset");
+    if (pr != user.end()) {
+        const char* t = preset_text(pr->second);
+        if (!t) {
+            delete s;
+            return OSE_EINVAL;
+        }
+        parse_into(s->p, t);
+    }
+    for (auto& kv : user) s->p[kv.first] = kv.second;
+    if (s->get("boundary", "") == "" && !s->p.count("bl")) s->p["boundary"] = "20";
+    *out = s;
+    return OSE_OK;
+}
+
+const char* oseh_sim_error(oseh_sim* s) { return s ? s->err.c_str() : "null simulation"; }
+
+int oseh_sim_build(oseh_sim* s)
+{
+    if (!s) return OSE_EINVAL;
+    if (!s->models.empty()) return OSE_OK;
+    int rc;
+    if ((rc = build_specs(s))) return rc;
+    // source wavelets (Alloc_wavelet, ose_3d_lib.cu:557-586)
+    const int nt = s->geti("nt", 300);
+    if (nt <= 0) return s->fail(OSE_EINVAL, "nt must be positive");
+    std::vector<double> w(nt), hh(nt);
+    oseh_ricker(w.data(), nt, s->getd("fm", 25.0), s->getd("dt", 1e-3), s->getd("amp", 100.0),
+                s->geti("delay", 40));
+    oseh_hilbert(w.data(), hh.data(), nt, s->get("hilbert", "unnorm") == "norm" ? 1 : 0);
+    s->s.assign(w.begin(), w.end());
+    s->h.assign(hh.begin(), hh.end());
+    s->models.clear();
+    for (auto& spec : s->specs) {
+        BuiltModel m;
+        if ((rc = build_model(s, spec, m))) return rc;
+        s->models.push_back(std::move(m));
+    }
+    s->shots.clear();
+    for (size_t mi = 0; mi < s->models.size(); ++mi) {
+        const BuiltModel& m = s->models[mi];
+        for (auto& sh : m.spec.shots) {
+            if (sh.x < 0 || sh.x >= m.spec.nx || sh.y < 0 || sh.y >= m.spec.ny || sh.z < 0 || sh.z >= m.spec.nz)
+                return s->fail(OSE_EINVAL, "source outside the interior");
+            s->shots.push_back(ShotRec{(int)mi, {m.bl + sh.x, m.bf + sh.y, m.bu + sh.z}});
+        }
+    }
+    return OSE_OK;
+}
+
+int oseh_sim_setup(oseh_sim* s)
+{
+    if (!s) return OSE_EINVAL;
+    if (s->ctx) return s->fail(OSE_ESTATE, "already set up");
+    int rc;
+    if ((rc = oseh_sim_build(s))) return rc;
+    const int nt = s->geti("nt", 300);
+    if ((rc = ose_create(&s->ctx, s->geti("device", 0)))) return s->fail(rc, "ose_create failed");
+    ose_set_options(s->ctx, s->geti("graph", 1), 0);
+    const int rec_z = s->geti("rec_z", 0);
+    for (size_t mi = 0; mi < s->models.size(); ++mi) {
+        BuiltModel& m = s->models[mi];
+        ose_model_desc d{};
+        d.nxb = m.nxb;
+        d.nyb = m.nyb;
+        d.nzb = m.nzb;
+        d.bl = m.bl;
+        d.bf = m.bf;
+        d.bu = m.bu;
+        d.nx = m.spec.nx;
+        d.ny = m.spec.ny;
+        d.nz = m.spec.nz;
+        d.dx = s->getd("dx", 10);
+        d.dy = s->getd("dy", 10);
+        d.dz = s->getd("dz", 10);
+        d.dt = s->getd("dt", 1e-3);
+        d.R = m.R;
+        d.K = m.K;
+        d.coef = m.coef.data();
+        d.taper_x = m.gx.data();
+        d.taper_y = m.gy.data();
+        d.taper_z = m.gz.data();
+        d.rec_z = rec_z;
+        if ((rc = ose_add_model(s->ctx, &d, m.v.data(), &m.id)))
+            return s->fail(rc, std::string("ose_add_model: ") + ose_last_error(s->ctx));
+    }
+    // shots in context order (ose_add_shot ids equal the shot indices)
+    for (const ShotRec& r : s->shots) {
+        BuiltModel& m = s->models[r.model];
+        int sid = -1;
+        if ((rc = ose_add_shot(s->ctx, m.id, r.src[0], r.src[1], r.src[2], s->s.data(), s->h.data(), nt, &sid)))
+            return s->fail(rc, std::string("ose_add_shot: ") + ose_last_error(s->ctx));
+        m.shot_ids.push_back(sid);
+    }
+    return OSE_OK;
+}
+
+ose_ctx* oseh_sim_ctx(oseh_sim* s) { return s ? s->ctx : nullptr; }
+
+int oseh_sim_run(oseh_sim* s, int nsteps, double* ms)
+{
+    if (!s) return OSE_EINVAL;
+    if (!s->ctx) return s->fail(OSE_ESTATE, "not set up");
+    double t = 0.0;
+    int rc = ose_timed_run(s->ctx, nsteps, &t);
+    if (rc) return s->fail(rc, std::string("ose_timed_run: ") + ose_last_error(s->ctx));
+    s->steps_done += nsteps;
+    if (ms) *ms = t;
+    return OSE_OK;
+}
+
+int oseh_sim_info(oseh_sim* s, char* buf, size_t cap)
+{
+    if (!s || !buf) return OSE_EINVAL;
+    std::ostringstream o;
+    double interior = 0, padded = 0;
+    o << "{\"models\": [";
+    for (size_t i = 0; i < s->models.size(); ++i) {
+        const BuiltModel& m = s->models[i];
+        const double ns = (double)m.spec.shots.size();
+        interior += ns * m.spec.nx * m.spec.ny * m.spec.nz;
+        padded += ns * (double)m.nxb * m.nyb * m.nzb;
+        o << (i ? ", " : "") << "{\"interior\": [" << m.spec.nx << ", " << m.spec.ny << ", " << m.spec.nz
+          << "], \"padded\": [" << m.nxb << ", " << m.nyb << ", " << m.nzb << "], \"pads\": [" << m.bl << ", "
+          << m.br << ", " << m.bf << ", " << m.bb << ", " << m.bu << ", " << m.bd << "], \"K\": " << m.K
+          << ", \"R\": " << m.R << ", \"z\": " << m.z << ", \"vmax\": " << m.vmax
+          << ", \"shots\": " << m.spec.shots.size() << "}";
+    }
+    o << "], \"nshots\": " << s->shots.size() << ", \"interior_cells\": " << (long long)interior
+      << ", \"padded_cells\": " << (long long)padded << ", \"nt\": " << s->geti("nt", 300)
+      << ", \"taper\": \"" << s->get("taper", "ref") << "\", \"korder\": \"" << s->get("korder", "tail")
+      << "\"}";
+    const std::string t = o.str();
+    if (t.size() + 1 > cap) return OSE_EINVAL;
+    std::memcpy(buf, t.c_str(), t.size() + 1);
+    return OSE_OK;
+}
+
+int oseh_sim_write(oseh_sim* s, const char* dir)
+{
+    if (!s || !dir) return OSE_EINVAL;
+    if (!s->ctx) return s->fail(OSE_ESTATE, "not set up");
+    const int nt = std::min(s->steps_done, s->geti("nt", 300));
+    const double dx = s->getd("dx", 10), dy = s->getd("dy", 10), dt = s->getd("dt", 1e-3);
+    int rc;
+    for (size_t k = 0; k < s->shots.size(); ++k) {
+        const BuiltModel& m = s->models[s->shots[k].model];
+        const int nrec = m.spec.nx * m.spec.ny;
+        std::vector<float> tr((size_t)nrec * nt);
+        if ((rc = ose_get_traces(s->ctx, (int)k, 0, nt, tr.data())))
+            return s->fail(rc, std::string("ose_get_traces: ") + ose_last_error(s->ctx));
+        const std::string base = std::string(dir) + "/shot" + std::to_string(k);
+        if ((rc = oseh_write_raw((base + ".bin").c_str(), tr.data(), tr.size())))
+            return s->fail(rc, "cannot write " + base + ".bin");
+        std::vector<double> rx(nrec), ry(nrec);
+        for (int iy = 0; iy < m.spec.ny; ++iy)
+            for (int ix = 0; ix < m.spec.nx; ++ix) {
+                rx[(size_t)iy * m.spec.nx + ix] = ix * dx;
+                ry[(size_t)iy * m.spec.nx + ix] = iy * dy;
+            }
+        const double sx = (s->shots[k].src[0] - m.bl) * dx, sy = (s->shots[k].src[1] - m.bf) * dy;
+        if (nt <= 65535 && (rc = oseh_write_su((base + ".su").c_str(), tr.data(), nrec, nt, dt, rx.data(),
+                                               ry.data(), sx, sy, (int)k)))
+            return s->fail(rc, "cannot write " + base + ".su");
+    }
+    if (s->geti("snap", 0) && (rc = oseh_sim_snapshot(s, dir))) return rc;
+    return OSE_OK;
+}
+
+void oseh_sim_destroy(oseh_sim* s)
+{
+    if (!s) return;
+    if (s->ctx) ose_destroy(s->ctx);
+    delete s;
+}
+
+int oseh_sim_snapshot(oseh_sim* s, const char* dir)
+{
+    if (!s || !dir) return OSE_EINVAL;
+    if (!s->ctx) return s->fail(OSE_ESTATE, "not set up");
+    int rc;
+    for (size_t k = 0; k < s->shots.size(); ++k) {
+        const BuiltModel& m = s->models[s->shots[k].model];
+        const size_t n = (size_t)m.nxb * m.nyb * m.nzb;
+        std::vector<float> re(n), im(n);
+        if ((rc = ose_get_wavefield(s->ctx, (int)k, re.data(), im.data())))
+            return s->fail(rc, std::string("ose_get_wavefield: ") + ose_last_error(s->ctx));
+        // ./wf/{p,q}-<it>-single.bin of ose_3d.cu:261-269, one pair per shot
+        const std::string tag = std::to_string(s->steps_done) + "-shot" + std::to_string(k) + ".bin";
+        if ((rc = oseh_write_snapshot((std::string(dir) + "/p-" + tag).c_str(), re.data(), m.nxb, m.nyb, m.nzb)) ||
+            (rc = oseh_write_snapshot((std::string(dir) + "/q-" + tag).c_str(), im.data(), m.nxb, m.nyb, m.nzb)))
+            return s->fail(rc, "cannot write snapshot");
+    }
+    return OSE_OK;
+}
+
+int oseh_sim_nmodels(oseh_sim* s) { return s ? (int)s->models.size() : 0; }
+
+int oseh_sim_model_dims(oseh_sim* s, int mi, int* d)
+{
+    if (!s || !d || mi < 0 || mi >= (int)s->models.size()) return OSE_EINVAL;
+    const BuiltModel& m = s->models[mi];
+    const int v[12] = {m.spec.nx, m.spec.ny, m.spec.nz, m.bl, m.br, m.bf, m.bb, m.bu, m.bd, m.nxb, m.nyb, m.nzb};
+    std::memcpy(d, v, sizeof v);
+    return OSE_OK;
+}
+
+int oseh_sim_model_velocity(oseh_sim* s, int mi, float* v)
+{
+    if (!s || !v || mi < 0 || mi >= (int)s->models.size()) return OSE_EINVAL;
+    std::copy(s->models[mi].v.begin(), s->models[mi].v.end(), v);
+    return OSE_OK;
+}
+
+int oseh_sim_model_params(oseh_sim* s, int mi, double* R, int* K, double* coef, float* gx, float* gy,
+                          float* gz, int* rec_z)
+{
+    if (!s || mi < 0 || mi >= (int)s->models.size()) return OSE_EINVAL;
+    const BuiltModel& m = s->models[mi];
+    if (R) *R = m.R;
+    if (K) *K = m.K;
+    if (coef) std::copy(m.coef.begin(), m.coef.end(), coef);
+    if (gx) std::copy(m.gx.begin(), m.gx.end(), gx);
+    if (gy) std::copy(m.gy.begin(), m.gy.end(), gy);
+    if (gz) std::copy(m.gz.begin(), m.gz.end(), gz);
+    if (rec_z) *rec_z = s->geti("rec_z", 0);
+    return OSE_OK;
+}
+
+int oseh_sim_nshots(oseh_sim* s) { return s ? (int)s->shots.size() : 0; }
+
+int oseh_sim_shot(oseh_sim* s, int k, int* model, int* src, int* nt, float* sw, float* hw)
+{
+    if (!s || k < 0 || k >= (int)s->shots.size()) return OSE_EINVAL;
+    if (model) *model = s->shots[k].model;
+    if (src) std::memcpy(src, s->shots[k].src, sizeof(int) * 3);
+    if (nt) *nt = (int)s->s.size();
+    if (sw) std::copy(s->s.begin(), s->s.end(), sw);
+    if (hw) std::copy(s->h.begin(), s->h.end(), hw);
+    return OSE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,175 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X OSE time stepper (one GPU).
+
+A step is one OSE time step (source injection, K operator applications with
+the Chebyshev recursion, taper, receiver gather) of every shot of the
+workload; inputs are resident in HBM before the timed region starts.
+
+    python bench.py [--steps K] [--warmup W] [--config c4|c1|c0|c2|c3|ref]
+
+Prints one JSON line.  value = interior cells x shots x steps / s (whole job,
+in Gcell*steps/s), timed with HIP events on the backend's stream.  The
+roofline entry is the dominant pass kernel's algorithmic bytes per launch over
+its mean HIP-event launch duration; the cpu_baseline entry times the FP64
+oracle (OpenMP, all host cores) on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "3d-ose_amd"))
+
+METRIC = ("Gcell·steps/s on 1 GPU at seismogram rel-L2≤1e-3 vs FP64; "
+          "learned vs plain pad")
+
+CONFIGS = {
+    # 768^3 x 8 shots, seeded 12-layer + dipping salt model, 40-cell taper
+    "c4": ("preset=c4", "768^3 x 8 shots, 12 layers + dipping salt, plain 40-cell taper"),
+    "c1": ("preset=c1", "256^3 single shot, 8 layers + 6 anomalies, 40-cell taper"),
+    "c0": ("preset=c0", "100^3 two-layer, 20-cell Cerjan taper"),
+    "c2": ("preset=c2", "16 shots on 512x512x384 + 4 uneven models, 40-cell taper"),
+    "c3": ("preset=c3", "256^3 single shot, thin 8-cell pad (taper)"),
+    "ref": ("preset=ref", "reference as shipped: 101^3 two-layer, 25-cell pads"),
+}
+PASS_BYTES_DOC = {
+    "fwd_y": "16 B/cell (read X_j, write S)",
+    "z_symbol": "16 B/cell (read S, write S)",
+    "inv_y": "16 B/cell (read S, write S)",
+    "x_epilogue": "44 B/cell (read S, X_{j-1}, v/R, A; write X_{j+1}, A)",
+    "x_first": "36 B/cell (read S, X_0, v/R; write X_1, A)",
+    "x_last": "36 B/cell (read S, X_{K-2}, v/R, A; write X_0)",
+    "inject": "16 B per x cell of the source line",
+}
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(sim, info, seconds):
+    """FP64 oracle on a centred cube of the workload's first model."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import pyoracle
+
+    m = sim.model(0)
+    d = m["dims"]
+    nxb, nyb, nzb = d[9], d[10], d[11]
+    side = min(nxb, nyb, nzb, 216)
+    z0, y0, x0 = (nzb - side) // 2, (nyb - side) // 2, (nxb - side) // 2
+    v = m["v"][z0:z0 + side, y0:y0 + side, x0:x0 + side].astype(np.float64)
+    om = pyoracle.Model(v, 10.0, 10.0, 10.0, 1e-3, m["R"], m["coef"], np.ones(side), np.ones(side),
+                        np.ones(side))
+    P = np.zeros(v.shape)
+    Q = np.zeros(v.shape)
+    P[side // 2, side // 2, side // 2] = 1.0
+    steps, t0 = 0, time.time()
+    while True:
+        P, Q = om.step(P, Q)
+        steps += 1
+        if time.time() - t0 > seconds or steps >= 8:
+            break
+    dt = time.time() - t0
+    padded_rate = side ** 3 * steps / dt / 1e9
+    ratio = info["interior_cells"] / info["padded_cells"]
+    return {
+        "value": padded_rate * ratio,
+        "unit": "Gcell*steps/s",
+        "cores": pyoracle.num_threads(),
+        "kind": "port",
+        "sample": (f"FP64 oracle, {steps} step(s) of one shot on a {side}^3 cube cut from the "
+                   f"workload's padded model (same K={m['K']}), {dt:.1f} s; padded-cell rate "
+                   f"{padded_rate:.6f} scaled by the workload's interior/padded ratio {ratio:.3f}"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c4", choices=sorted(CONFIGS))
+    ap.add_argument("--extra", default="", help="extra key=value parameters")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); shots shard "
+              "across GPUs without exchange, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    from ose_amd.host import Simulation
+
+    params, workload = CONFIGS[args.config]
+    prof_steps = max(2, min(args.steps, 4))
+    nt = args.warmup + args.steps + prof_steps
+    sim = Simulation(f"{params} nt={nt} {args.extra}")
+    t0 = time.time()
+    sim.setup()
+    info = sim.info()
+    print(f"[bench] set-up {time.time() - t0:.1f} s: {json.dumps(info)}", file=sys.stderr, flush=True)
+
+    # warm-up (also builds the per-step hipGraph)
+    sim.timed(args.warmup)
+    # timed region: K graph-replayed steps between stream synchronisations
+    ms = sim.timed(args.steps)
+    print(f"[bench] timed {args.steps} steps: {ms:.1f} ms", file=sys.stderr, flush=True)
+    # per-pass HIP-event durations over the following steps (eager launches)
+    sim.set_options(use_graph=False, profile=True)
+    sim.reset_stats()
+    ms_prof = sim.timed(prof_steps)
+    stats = sim.pass_stats()
+    sim.set_options(use_graph=True, profile=False)
+
+    cells = info["interior_cells"]
+    value = cells * args.steps / (ms * 1e-3) / 1e9
+    passes = {}
+    for name, (avg_ms, n, by) in stats.items():
+        if n:
+            passes[name] = {"avg_us": avg_ms * 1e3, "launches": n, "bytes": by,
+                            "GBs": by / (avg_ms * 1e-3) / 1e9 if avg_ms else 0.0}
+    dom = max((p for p in passes if p != "inject"), key=lambda p: passes[p]["avg_us"] * passes[p]["launches"])
+    pd = passes[dom]
+    roofline = {"bound": "hbm", "achieved": pd["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": pd["GBs"] / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                "bytes_per_launch": pd["bytes"], "bytes_rule": PASS_BYTES_DOC[dom],
+                "passes": {k: round(v["GBs"], 1) for k, v in passes.items()}}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(sim, info, args.cpu_seconds)
+    m0 = info["models"][0]
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "Gcell*steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded velocity model, Ricker source)",
+        "config": {
+            "workload": workload,
+            "preset": args.config,
+            "boundary": "taper",
+            "interior": m0["interior"],
+            "padded": m0["padded"],
+            "K": m0["K"],
+            "shots": info["nshots"],
+            "interior_cells_per_step": cells,
+            "padded_cells_per_step": info["padded_cells"],
+            "padded_rate": info["padded_cells"] * args.steps / (ms * 1e-3) / 1e9,
+            "ms_per_step_profiled": ms_prof / prof_steps,
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    sim.close()
+
+
+if __name__ == "__main__":
+    main()

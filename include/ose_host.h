@@ -1,0 +1,139 @@
+/*
+ * ose_host.h -- C++ host library of the OSE driver (libose_host.so).
+ *
+ * Everything the reference's main() and ose_3d_lib.cu do on the host (or in
+ * <<<1,1>>> kernels) around the time step, in FP64: grid and pad rules,
+ * Ricker + Hilbert source, Bessel coefficients, order rule, taper profiles,
+ * velocity file I/O and padding, seeded models, seismogram/.su/snapshot
+ * output, and a simulation object that drives include/ose_api.h.  It
+ * reaches the GPU only through that C-ABI.
+ */
+#ifndef OSE_HOST_H
+#define OSE_HOST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "ose_api.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- grid ------------------------------------------------------------ */
+/* judge_odd (ose_3d_lib.cu:36-44): odd padded extent -> after-pad + 1.     */
+int oseh_judge_odd(int* after, int* nb);
+
+/* ---- source ---------------------------------------------------------- */
+/* generate_wavelet_gpu (ose_3d_kernel.cu:3-13), FP64.                     */
+void oseh_ricker(double* w, int nt, double fm, double dt, double amp, int delay);
+/* R2C -> hilbert_1d -> C2R (ose_3d_lib.cu:575-583, ose_3d_kernel.cu:154-172).
+ * normalize = 0: the reference's per-shot copy (times nt, quirk Q4);
+ * normalize = 1: main()'s scaled copy (scale_wavelet, ose_3d.cu:179).      */
+void oseh_hilbert(const double* s, double* h, int nt, int normalize);
+
+/* ---- Bessel / order -------------------------------------------------- */
+/* J_n(x), n = 0..nmax, by Miller's backward recursion normalised with
+ * J0 + 2 sum J_2k = 1 (replaces jnf, ose_3d_kernel.cu:310,388).            */
+int oseh_bessel_all(double x, int nmax, double* J);
+/* coef[0] = J0(z), coef[k] = 2 J_k(z), k = 1..K.                           */
+int oseh_coefficients(double z, int K, double* coef);
+/* R = vmax * pi * sqrt(dx^-2 + dy^-2 + dz^-2)  (ose_3d.cu:194).            */
+double oseh_spectral_radius(double vmax, double dx, double dy, double dz);
+/* Reference order rule: M = (int)(R dt) + 6, K = M + 1 (ose_3d.cu:197).    */
+int oseh_order_reference(double z);
+/* Smallest K with 2 sum_{k>K} |J_k(z)| < tol.                              */
+int oseh_order_tail(double z, double tol);
+
+/* ---- taper ----------------------------------------------------------- */
+#define OSEH_TAPER_REF    0   /* generate_att (ose_3d_kernel.cu:242-261)      */
+#define OSEH_TAPER_CERJAN 1   /* exp(-(alpha d)^2), d = 1..b from interior    */
+#define OSEH_TAPER_NONE   2
+void oseh_taper(float* a, int before, int nn, int after, int mode, double alpha);
+
+/* ---- velocity -------------------------------------------------------- */
+/* Read a reference velocity file ([x][y][z], z fastest, float32 LE;
+ * input_file_xyz_boundary ose_3d_lib.cu:887-909) into interior [z][y][x]. */
+int oseh_read_velocity_xyz(const char* path, int nx, int ny, int nz, float* v_zyx);
+/* Edge-replicate an interior cube into the padded grid
+ * (add_pml_layers_v_h ose_3d_lib.cu:912-1032).                            */
+void oseh_pad_velocity(const float* v_int, int nx, int ny, int nz, int bl, int br, int bf,
+                       int bb, int bu, int bd, float* v_pad);
+/* Seeded generators, interior [nz][ny][nx]:
+ *   kind 0: constant v0
+ *   kind 1: two layers, v0 above depth index `iface`, v1 below
+ *   kind 2: 8 flat jittered layers 1500..4500 m/s + 6 Gaussian anomalies
+ *           (sigma 15 cells, +-15 %), 5-cell box smoothing
+ *   kind 3: 12 jittered layers 1500..4000 m/s + a dipping ellipsoidal salt
+ *           body (4500 m/s, semi-axes 150/120/80 cells scaled to the grid)
+ *   kind 4: layered random model for boundary training (3-9 layers,
+ *           0-4 anomalies)                                                 */
+int oseh_model(int kind, int nx, int ny, int nz, uint64_t seed, double v0, double v1,
+               int iface, float* v_zyx);
+uint64_t oseh_rng_next(uint64_t* state);   /* xoshiro256** over a 4-word state */
+double oseh_rng_uniform(uint64_t* state);
+double oseh_rng_normal(uint64_t* state);
+void oseh_rng_seed(uint64_t* state, uint64_t seed);
+
+/* ---- output ---------------------------------------------------------- */
+/* Raw trace-major float32 seismogram [nrec][nt] (write_3dfile_from_1darray,
+ * ose_3d_lib.cu:327-343, called at ose_3d.cu:276).                         */
+int oseh_write_raw(const char* path, const float* data, size_t n);
+/* SU file: per trace a 240-byte SEG-Y trace header (native endianness:
+ * tracl @0, tracr @4, fldr @8, tracf @12, ep @16, cdp @20, sx @72, sy @76,
+ * gx @80, gy @84, scalco @70, ns @114 (u16), dt @116 (u16, microseconds))
+ * followed by nt float32 samples.  Coordinates in metres times 100
+ * (scalco = -100).  rec_x/rec_y: receiver coordinates (m), src_x/src_y.   */
+int oseh_write_su(const char* path, const float* traces, int nrec, int nt, double dt,
+                  const double* rec_x, const double* rec_y, double src_x, double src_y,
+                  int shot_index);
+/* Read back an SU file written above: returns nrec, nt and the samples.   */
+int oseh_read_su(const char* path, float* traces, size_t cap, int* nrec, int* nt, double* dt);
+/* Wavefield snapshot over the padded grid, file order [y][x][z] with z
+ * fastest (write_3dfile_from_1darray_ci ose_3d_lib.cu:364-380).            */
+int oseh_write_snapshot(const char* path, const float* f_zyx, int nxb, int nyb, int nzb);
+
+/* ---- simulation ------------------------------------------------------ */
+/* A simulation is built from SU-style "key=value" parameters (the reference
+ * hard-codes them in main(), ose_3d.cu:75-125) or a preset:
+ *   preset=ref    the reference as shipped (101^3 from a velocity file or the
+ *                 equivalent 3000/4000 two-layer model, 25-cell pads, nt=300)
+ *   preset=c0..c4 the five benchmark configurations.
+ * Keys: nx ny nz dx dy dz dt nt fm amp delay boundary bl br bf bb bu bd
+ *       model seed v0 v1 iface vfile sx sy sz nshot jsx jsy jsz srcgrid
+ *       rec_z korder ktol taper alpha extent hilbert device graph
+ *       max_shots (cap on shots per model) snap                            */
+typedef struct oseh_sim oseh_sim;
+int  oseh_sim_create(oseh_sim** out, const char* params);
+/* Build velocities, pads, wavelets, orders and coefficients (host only).  */
+int  oseh_sim_build(oseh_sim* sim);
+/* oseh_sim_build, then register everything on a new ose_ctx
+ * (ose_add_model / ose_add_shot).                                          */
+int  oseh_sim_setup(oseh_sim* sim);
+/* Run nsteps for every shot; ms receives the HIP-event time of the loop.  */
+int  oseh_sim_run(oseh_sim* sim, int nsteps, double* ms);
+ose_ctx* oseh_sim_ctx(oseh_sim* sim);
+/* JSON description: models (interior, padded, K, R, taper), shots,
+ * interior and padded cells per step.                                      */
+int  oseh_sim_info(oseh_sim* sim, char* buf, size_t cap);
+/* Write traces (raw + .su) and, if requested, snapshots into dir.          */
+int  oseh_sim_write(oseh_sim* sim, const char* dir);
+/* Write the current wavefield of every shot as p-<it>-shot<k>.bin (Re u)
+ * and q-<it>-shot<k>.bin (Im u) in the snapshot order (ose_3d.cu:260-270). */
+int  oseh_sim_snapshot(oseh_sim* sim, const char* dir);
+const char* oseh_sim_error(oseh_sim* sim);
+void oseh_sim_destroy(oseh_sim* sim);
+/* Model / shot data of a simulation, for checking against the oracle.    */
+int  oseh_sim_nmodels(oseh_sim* sim);
+int  oseh_sim_model_dims(oseh_sim* sim, int model, int* dims /* 12: nx ny nz bl br bf bb bu bd nxb nyb nzb */);
+int  oseh_sim_model_velocity(oseh_sim* sim, int model, float* v_pad);
+int  oseh_sim_model_params(oseh_sim* sim, int model, double* R, int* K, double* coef /* K+1 */,
+                           float* gx, float* gy, float* gz, int* rec_z);
+int  oseh_sim_nshots(oseh_sim* sim);
+int  oseh_sim_shot(oseh_sim* sim, int shot, int* model, int* src /* padded xyz */, int* nt,
+                   float* s, float* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OSE_HOST_H */

@@ -1,0 +1,46 @@
+"""The C-ABI libraries load and export every entry point their headers
+declare (CPU only; no device calls)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "3d-ose_amd", "lib")
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ose[h]?_\w+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("lib,header", [("libose_hip.so", "ose_api.h"), ("libose_host.so", "ose_host.h")])
+def test_exports_every_declared_symbol(lib, header):
+    L = ctypes.CDLL(os.path.join(LIB, lib))
+    names = declared(header)
+    assert len(names) > 10
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_python_binding_lists_the_header():
+    from ose_amd import _lib
+    assert sorted(_lib.hip_symbols()) == declared("ose_api.h")
+    from ose_amd import _host_protos
+    assert sorted(_host_protos.PROTOS) == declared("ose_host.h")
+
+
+def test_host_only_entry_points():
+    from ose_amd import _lib
+    L = _lib.hip()
+    assert L.ose_destroy(None) == _lib.OSE_EINVAL
+    assert L.ose_last_error(None) == b"null context"
+    assert L.ose_extent_is_supported(152) and L.ose_extent_is_supported(864)
+    assert not L.ose_extent_is_supported(17)
+    assert L.ose_supported_extent(848) == 864
+    assert L.ose_supported_extent(272) == 280
+    assert L.ose_supported_extent(141) == 144
+    assert L.ose_add_model(None, None, None, None) == _lib.OSE_EINVAL
+    assert L.ose_run(None, 1) == _lib.OSE_EINVAL

@@ -1,0 +1,176 @@
+"""The C++ host library (libose_host.so) against the oracle and fixtures.
+CPU only: nothing here touches the device."""
+import os
+import struct
+
+import numpy as np
+import pytest
+
+import pyoracle as orc
+from _cases import rel
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+REF_INPUT = "/root/reference/input"
+
+
+@pytest.fixture(scope="module")
+def H():
+    from ose_amd import host
+    return host
+
+
+def test_bessel_miller_matches_scipy(H):
+    B = np.load(os.path.join(GOLD, "bessel.npz"))
+    for j, x in enumerate(B["x"]):
+        J = H.bessel_all(float(x), 24)
+        assert np.abs(J - B["J"][:, j]).max() < 1e-14
+
+
+def test_bessel_matches_glibc_jn(H):
+    for x in (1e-3, 0.7, 2.177, 2.449, 9.5, 20.0):
+        J = H.bessel_all(x, 30)
+        ref = np.array([orc.bessel_jn(n, x) for n in range(31)])
+        assert np.abs(J - ref).max() < 1e-14
+
+
+def test_coefficients_and_orders(H):
+    z = 2.449
+    c = H.coefficients(z, 11)
+    assert abs(c[0] - orc.bessel_jn(0, z)) < 1e-15
+    assert all(abs(c[k] - 2 * orc.bessel_jn(k, z)) < 1e-15 for k in range(1, 12))
+    # reference rule M = (int)z + 6, K = M + 1 (ose_3d.cu:197)
+    assert H.order_reference(2.177) == 9
+    assert H.order_reference(1.904) == 8
+    # tail rule 2 sum_{k>K}|J_k(z)| < 1e-7
+    for z in (1.904, 2.177, 2.449, 2.816):
+        K = H.order_tail(z, 1e-7)
+        tail = lambda K: 2 * sum(abs(orc.bessel_jn(k, z)) for k in range(K + 1, K + 60))
+        assert tail(K) < 1e-7 <= tail(K - 1)
+    R = H.spectral_radius(4000.0, 10, 10, 10)
+    assert abs(R - 4000 * np.pi * np.sqrt(0.03)) < 1e-9
+
+
+def test_wavelet_matches_oracle(H):
+    for nt, fm in ((300, 40.0), (1000, 25.0), (301, 20.0)):
+        s = H.ricker(nt, fm, 1e-3)
+        assert np.abs(s - orc.ricker(nt, fm, 1e-3)).max() < 1e-12
+        h = H.hilbert(s)
+        assert rel(h, orc.hilbert(s)) < 1e-12
+        assert rel(H.hilbert(s, normalize=True), h / nt) < 1e-14
+
+
+def test_taper_matches_oracle(H):
+    assert np.abs(H.taper(25, 101, 26) - orc.taper_ref(25, 101, 26)).max() < 1e-7
+    assert np.abs(H.taper(20, 100, 20, "cerjan", 0.015) - orc.taper_cerjan(20, 100, 20, 0.015)).max() < 1e-7
+    assert np.all(H.taper(3, 4, 5, "none") == 1.0)
+    assert H.judge_odd(25, 151) == (26, 152)
+
+
+def test_pad_velocity_edge_mode(H):
+    rng = np.random.default_rng(3)
+    vi = rng.uniform(1500, 4500, (7, 5, 6)).astype(np.float32)
+    pads = (2, 3, 4, 1, 3, 2)
+    got = H.pad_velocity(vi, pads)
+    ref = np.pad(vi, ((3, 2), (4, 1), (2, 3)), mode="edge")
+    assert np.array_equal(got, ref)
+    o = orc.velocity_pad_interior(vi, 2, 4, 3, 6 + 5, 5 + 5, 7 + 5)
+    assert np.array_equal(got.astype(np.float64), o)
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_INPUT), reason="reference inputs not mounted")
+def test_reference_velocity_file_equals_twolayer_generator(H):
+    v = H.read_velocity_xyz(os.path.join(REF_INPUT, "v3d_nx101_ny101_nz101.bin"), 101, 101, 101)
+    g = H.model("twolayer", 101, 101, 101, v0=3000.0, v1=4000.0, iface=30)
+    assert np.array_equal(v, g)
+    # the other two reference inputs parse with their own dimensions
+    v2 = H.read_velocity_xyz(os.path.join(REF_INPUT, "v3d_nx100_ny101_nz102.bin"), 100, 101, 102)
+    assert set(np.unique(v2)) == {3000.0, 4000.0}
+    v3 = H.read_velocity_xyz(os.path.join(REF_INPUT, "3d_number-100-100-100-3.bin"), 100, 100, 100)
+    assert v3.min() == 6000.0 and v3.max() == 8500.0
+
+
+def test_read_velocity_rejects_short_file(H, tmp_path):
+    p = tmp_path / "short.bin"
+    np.zeros(10, dtype=np.float32).tofile(p)
+    with pytest.raises(Exception):
+        H.read_velocity_xyz(str(p), 4, 4, 4)
+
+
+def test_generators_are_seeded_and_in_range(H):
+    a = H.model("layered8", 48, 40, 32, seed=2)
+    b = H.model("layered8", 48, 40, 32, seed=2)
+    c = H.model("layered8", 48, 40, 32, seed=3)
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    assert 1500 * 0.8 < a.min() and a.max() < 4500 * 1.2
+    # velocity increases with depth on average
+    assert a[:4].mean() < a[-4:].mean()
+    s = H.model("salt12", 64, 64, 64, seed=5)
+    assert (s == 4500.0).sum() > 0 and s.min() >= 1500.0
+
+
+def test_su_round_trip_and_header(H, tmp_path):
+    rng = np.random.default_rng(4)
+    tr = rng.standard_normal((6, 37)).astype(np.float32)
+    p = str(tmp_path / "x.su")
+    H.write_su(p, tr, 1e-3, rec_x=np.arange(6) * 10.0, rec_y=np.zeros(6), src=(25.0, 0.0), shot=2)
+    got, dt = H.read_su(p)
+    assert np.array_equal(got, tr) and abs(dt - 1e-3) < 1e-12
+    raw = open(p, "rb").read()
+    assert len(raw) == 6 * (240 + 4 * 37)
+    h = raw[240 + 4 * 37:][:240]   # second trace
+    assert struct.unpack_from("<i", h, 0)[0] == 2          # tracl
+    assert struct.unpack_from("<i", h, 8)[0] == 3          # fldr = shot + 1
+    assert struct.unpack_from("<h", h, 70)[0] == -100      # scalco
+    assert struct.unpack_from("<i", h, 80)[0] == 1000      # gx = 10 m * 100
+    assert struct.unpack_from("<H", h, 114)[0] == 37       # ns
+    assert struct.unpack_from("<H", h, 116)[0] == 1000     # dt (us)
+
+
+def test_snapshot_order(H, tmp_path):
+    f = np.arange(2 * 3 * 4, dtype=np.float32).reshape(2, 3, 4)   # [z][y][x]
+    p = str(tmp_path / "s.bin")
+    H.write_snapshot(p, f)
+    got = np.fromfile(p, dtype=np.float32).reshape(3, 4, 2)       # [y][x][z]
+    assert np.array_equal(got, f.transpose(1, 2, 0))
+
+
+@pytest.mark.parametrize("preset,padded,K", [
+    ("preset=ref", (152, 152, 152), 9),
+    ("preset=c0", (140, 140, 140), None),
+    ("preset=c1", (336, 336, 336), None),
+    ("preset=c3", (280, 280, 280), None),
+])
+def test_presets_build(H, preset, padded, K):
+    sim = H.Simulation(preset).build()
+    d = sim.model_dims(0)
+    assert tuple(d[9:12]) == padded
+    m = sim.model(0)
+    if K is not None:
+        assert m["K"] == K
+    # the padded velocity is the edge-replicated interior
+    vi = m["v"][d[7]:d[7] + d[2], d[5]:d[5] + d[1], d[3]:d[3] + d[0]]
+    assert np.array_equal(m["v"], np.pad(vi, ((d[7], d[8]), (d[5], d[6]), (d[3], d[4])), mode="edge"))
+    assert abs(m["coef"][0] - orc.bessel_jn(0, m["R"] * 1e-3)) < 1e-14
+    sim.close()
+
+
+def test_reference_preset_parameters(H):
+    sim = H.Simulation("preset=ref").build()
+    d = sim.model_dims(0)
+    assert d[:9] == [101, 101, 101, 25, 26, 25, 26, 25, 26]
+    m = sim.model(0)
+    assert np.abs(m["gx"] - orc.taper_ref(25, 101, 26)).max() < 1e-7
+    assert abs(m["R"] - 4000 * np.pi * np.sqrt(0.03)) < 1e-9
+    sh = sim.shot(0)
+    assert sh["src"] == [75, 75, 25] and sh["nt"] == 300
+    assert np.abs(sh["s"] - orc.ricker(300, 40.0, 1e-3)).max() < 1e-4
+    sim.close()
+
+
+def test_c2_batch_layout(H):
+    sim = H.Simulation("preset=c2").build()
+    assert sim.nmodels() == 5
+    assert sim.nshots() == 20
+    pads = [sim.model_dims(i)[9:12] for i in range(5)]
+    assert pads[0] == [600, 600, 480]
+    sim.close()
