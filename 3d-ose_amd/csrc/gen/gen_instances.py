@@ -32,37 +32,75 @@ SIZES = {
     864: (27, 32), 960: (30, 32), 1024: (32, 32),
 }
 
-STRIDED_T = 16
+STRIDED_TS = (8, 16, 32)
+LDS_MAX = 160 * 1024
 
 
-def x_lines(nt):
-    return max(1, 256 // nt)
+def strided_variants(n1, n2):
+    nt = max(n1, n2)
+    out = []
+    for t in STRIDED_TS:
+        if t * nt <= 1024 and 8 * n1 * n2 * (t + 1) <= LDS_MAX:
+            out.append(t)
+    return out
 
 
-INST = """// Generated by csrc/gen/gen_instances.py -- do not edit.
+def x_variants(n1, n2):
+    nt = max(n1, n2)
+    pn = n1 | 1
+    out = []
+    for p in sorted({max(1, 128 // nt), max(1, 256 // nt)}):
+        if p * nt <= 1024 and 8 * p * n2 * pn <= LDS_MAX:
+            out.append(p)
+    return out
+
+
+HEAD = """// Generated by csrc/gen/gen_instances.py -- do not edit.
 #include "ose_kernels.h"
 #include "ose_entry.h"
 
 namespace ose {{
 void fill_entry_{N}(FftEntry& e)
 {{
-    constexpr int N1 = {N1}, N2 = {N2}, T = {T}, P = {P};
-    using SC = StridedCfg<N1, N2, T>;
-    using XC = XCfg<N1, N2, P>;
+    constexpr int N1 = {N1}, N2 = {N2};
     e.N = {N}; e.N1 = N1; e.N2 = N2;
-    e.s_T = T; e.s_threads = SC::THREADS; e.s_lds = SC::LDS_BYTES;
-    e.ystr[0] = (const void*)&k_strided<N1, N2, T, -1>;
-    e.ystr[1] = (const void*)&k_strided<N1, N2, T, 1>;
-    e.zsym = (const void*)&k_zsym<N1, N2, T>;
-    e.x_P = P; e.x_threads = XC::THREADS; e.x_lds = XC::LDS_BYTES;
-    e.xk[XM_ORDER0] = (const void*)&k_xpass<N1, N2, P, XM_ORDER0>;
-    e.xk[XM_MID] = (const void*)&k_xpass<N1, N2, P, XM_MID>;
-    e.xk[XM_LAST] = (const void*)&k_xpass<N1, N2, P, XM_LAST>;
-    e.xk[XM_FWD] = (const void*)&k_xpass<N1, N2, P, XM_FWD>;
-    e.xk[XM_INV] = (const void*)&k_xpass<N1, N2, P, XM_INV>;
-}}
+"""
+STRIDED = """    {{
+        constexpr int T = {T};
+        using SC = StridedCfg<N1, N2, T>;
+        StridedKern k;
+        k.T = T; k.threads = SC::THREADS; k.lds = SC::LDS_BYTES;
+        k.fn = (const void*)&k_strided<N1, N2, T, -1>; e.yf[e.ns] = k;
+        k.fn = (const void*)&k_strided<N1, N2, T, 1>; e.yi[e.ns] = k;
+        k.fn = (const void*)&k_zsym<N1, N2, T>; e.zs[e.ns] = k;
+        ++e.ns;
+    }}
+"""
+XPASS = """    {{
+        constexpr int P = {P};
+        using XC = XCfg<N1, N2, P>;
+        XKern k;
+        k.P = P; k.threads = XC::THREADS; k.lds = XC::LDS_BYTES;
+        k.fn[XM_ORDER0] = (const void*)&k_xpass<N1, N2, P, XM_ORDER0>;
+        k.fn[XM_MID] = (const void*)&k_xpass<N1, N2, P, XM_MID>;
+        k.fn[XM_LAST] = (const void*)&k_xpass<N1, N2, P, XM_LAST>;
+        k.fn[XM_FWD] = (const void*)&k_xpass<N1, N2, P, XM_FWD>;
+        k.fn[XM_INV] = (const void*)&k_xpass<N1, N2, P, XM_INV>;
+        e.xk[e.nx++] = k;
+    }}
+"""
+TAIL = """}}
 }}  // namespace ose
 """
+
+
+def instance_source(n, n1, n2):
+    src = HEAD.format(N=n, N1=n1, N2=n2)
+    for t in strided_variants(n1, n2):
+        src += STRIDED.format(T=t)
+    for p in x_variants(n1, n2):
+        src += XPASS.format(P=p)
+    return src + TAIL.format()
 
 
 def main():
@@ -79,8 +117,7 @@ def main():
             f.write(f"OSE_SIZE({n})\n")
     for n in sorted(sizes):
         n1, n2 = sizes[n]
-        nt = max(n1, n2)
-        src = INST.format(N=n, N1=n1, N2=n2, T=STRIDED_T, P=x_lines(nt))
+        src = instance_source(n, n1, n2)
         path = os.path.join(out, f"inst_{n}.hip")
         old = open(path).read() if os.path.exists(path) else None
         if old != src:

@@ -15,7 +15,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -115,6 +117,9 @@ struct Model {
     std::vector<double> coef;
     std::vector<float> gx, gy, gz;
     const FftEntry *ex = nullptr, *ey = nullptr, *ez = nullptr;
+    // chosen launch shapes (index into the entry's variants)
+    int sel_yf = 0, sel_z = 0, sel_yi = 0, sel_x = 0;
+    bool tuned = false;
     long ncell = 0;
     DevBuf vr, g, k2, tw;          // g: gx|gy|gz, k2: kx2|ky2|kz2, tw: twx|twy|twz
     float* d_vr = nullptr;
@@ -159,6 +164,7 @@ struct ose_ctx {
     bool finalized = false;
     bool use_graph = true;
     bool profile = false;
+    bool autotune = true;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     std::string err;
@@ -279,12 +285,11 @@ int collect(ose_ctx* c)
 }
 
 // strided pass over the y axis (es = nx) or the z axis (es = nx*ny)
-int launch_strided(ose_ctx* c, int kind, const FftEntry* e, const void* fn, const float2* in,
-                   float2* out, int nx, int ny, int nz, bool zaxis, int nshot, long shot_stride,
-                   const float2* tw, const Model* m, double bytes)
+int launch_strided(ose_ctx* c, int kind, const StridedKern& k, const float2* in, float2* out, int nx,
+                   int ny, int nz, bool zaxis, int nshot, long shot_stride, const float2* tw,
+                   const Model* m, double bytes)
 {
-    if (!in || !out || !tw || (fn == e->zsym && !m))
-        return c->fail(OSE_EINVAL, "strided pass launched with a missing buffer");
+    if (!k.fn || !in || !out || !tw) return c->fail(OSE_EINVAL, "strided pass launched with a missing buffer");
     StridedArgs a{};
     a.in = in;
     a.out = out;
@@ -299,16 +304,16 @@ int launch_strided(ose_ctx* c, int kind, const FftEntry* e, const void* fn, cons
         a.kz2 = m->d_kz2;
         a.invN = (float)(1.0 / ((double)nx * ny * nz));
     }
-    dim3 grid((nx + e->s_T - 1) / e->s_T, zaxis ? ny : nz, nshot);
-    return launch(c, kind, bytes, fn, grid, dim3(e->s_threads), e->s_lds, &a);
+    dim3 grid((nx + k.T - 1) / k.T, zaxis ? ny : nz, nshot);
+    return launch(c, kind, bytes, k.fn, grid, dim3(k.threads), k.lds, &a);
 }
 
-int launch_x(ose_ctx* c, int kind, const FftEntry* e, int mode, XArgs& a, int nlines, int nshot,
+int launch_x(ose_ctx* c, int kind, const XKern& k, int mode, XArgs& a, int nlines, int nshot,
              double bytes)
 {
     a.nlines = nlines;
     // every pointer the mode dereferences must be set before launching
-    bool ok = a.tw != nullptr;
+    bool ok = a.tw != nullptr && k.fn[mode] != nullptr;
     switch (mode) {
     case XM_FWD: ok = ok && a.A && a.Xout; break;
     case XM_INV: ok = ok && a.A && a.Xprev; break;
@@ -320,20 +325,21 @@ int launch_x(ose_ctx* c, int kind, const FftEntry* e, int mode, XArgs& a, int nl
     default: ok = false;
     }
     if (!ok) return c->fail(OSE_EINVAL, "x pass mode %d launched with a missing buffer", mode);
-    dim3 grid((nlines + e->x_P - 1) / e->x_P, nshot);
-    return launch(c, kind, bytes, e->xk[mode], grid, dim3(e->x_threads), e->x_lds, &a);
+    dim3 grid((nlines + k.P - 1) / k.P, nshot);
+    return launch(c, kind, bytes, k.fn[mode], grid, dim3(k.threads), k.lds, &a);
 }
 
 int prepare_kernels(ose_ctx* c, const FftEntry* e)
 {
-    // allow > 64 KiB of dynamic LDS where a split needs it
-    const void* fs[] = {e->ystr[0], e->ystr[1], e->zsym};
-    for (auto f : fs)
-        if (e->s_lds > 65536)
-            HIPCHK(c, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->s_lds));
-    for (auto f : e->xk)
-        if (e->x_lds > 65536)
-            HIPCHK(c, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->x_lds));
+    // allow > 64 KiB of dynamic LDS where a launch shape needs it
+    for (int v = 0; v < e->ns; ++v)
+        for (const StridedKern* k : {&e->yf[v], &e->yi[v], &e->zs[v]})
+            if (k->lds > 65536)
+                HIPCHK(c, hipFuncSetAttribute(k->fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)k->lds));
+    for (int v = 0; v < e->nx; ++v)
+        for (const void* f : e->xk[v].fn)
+            if (e->xk[v].lds > 65536)
+                HIPCHK(c, hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)e->xk[v].lds));
     return OSE_OK;
 }
 
@@ -381,13 +387,13 @@ int enqueue_model_step(ose_ctx* c, Model& m, int* d_it)
     const int K = m.d.K;
     for (int j = 0; j < K; ++j) {
         float2* Xj = m.X[j % 2];
-        if ((rc = launch_strided(c, PK_YF, m.ey, m.ey->ystr[0], Xj, m.S, nx, ny, nz, false, nshot,
+        if ((rc = launch_strided(c, PK_YF, m.ey->yf[m.sel_yf], Xj, m.S, nx, ny, nz, false, nshot,
                                  m.ncell, m.d_twy, nullptr, 16.0 * cells)))
             return rc;
-        if ((rc = launch_strided(c, PK_Z, m.ez, m.ez->zsym, m.S, m.S, nx, ny, nz, true, nshot,
+        if ((rc = launch_strided(c, PK_Z, m.ez->zs[m.sel_z], m.S, m.S, nx, ny, nz, true, nshot,
                                  m.ncell, m.d_twz, &m, 16.0 * cells)))
             return rc;
-        if ((rc = launch_strided(c, PK_YI, m.ey, m.ey->ystr[1], m.S, m.S, nx, ny, nz, false, nshot,
+        if ((rc = launch_strided(c, PK_YI, m.ey->yi[m.sel_yi], m.S, m.S, nx, ny, nz, false, nshot,
                                  m.ncell, m.d_twy, nullptr, 16.0 * cells)))
             return rc;
         XArgs a = base_xargs(m, nshot);
@@ -427,7 +433,7 @@ int enqueue_model_step(ose_ctx* c, Model& m, int* d_it)
                 bytes = 44.0 * cells;   // S, Xprev, vr, A read; X', A written
             }
         }
-        if ((rc = launch_x(c, kind, m.ex, mode, a, ny * nz, nshot, bytes))) return rc;
+        if ((rc = launch_x(c, kind, m.ex->xk[m.sel_x], mode, a, ny * nz, nshot, bytes))) return rc;
     }
     return OSE_OK;
 }
@@ -438,6 +444,73 @@ int enqueue_step(ose_ctx* c)
     for (auto& m : c->models)
         if ((rc = enqueue_model_step(c, *m, c->d_it))) return rc;
     return launch(c, -1, 0.0, (const void*)&k_advance, dim3(1), dim3(64), 0, &c->d_it);
+}
+
+// Launch-shape autotuner: time every compiled variant of each pass on the
+// model's own (still zero) buffers and keep the fastest.  Linear passes on
+// zero fields leave them zero, so the state is unchanged.
+int time_variant(ose_ctx* c, const std::function<int()>& body, int reps, float* ms)
+{
+    int rc;
+    if ((rc = body())) return rc;   // warm-up
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps && !rc; ++r) rc = body();
+    HIPCHK(c, hipEventRecord(e1, c->stream));
+    HIPCHK(c, hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = t / reps;
+    return rc;
+}
+
+int autotune_model(ose_ctx* c, Model& m)
+{
+    const int nshot = (int)m.shots.size();
+    if (m.tuned || nshot == 0) return OSE_OK;
+    const int nx = m.d.nxb, ny = m.d.nyb, nz = m.d.nzb;
+    const bool prof = c->profile;
+    c->profile = false;
+    int rc = OSE_OK;
+    float best, t;
+    // keep each timed region around a few milliseconds
+    const int reps = std::max(1, std::min(5, (int)(2e8 / ((double)m.ncell * nshot))));
+    best = 1e30f;
+    for (int v = 0; v < m.ey->ns && !rc; ++v) {
+        rc = time_variant(c, [&] { return launch_strided(c, -1, m.ey->yf[v], m.X[0], m.S, nx, ny, nz, false, nshot,
+                                                         m.ncell, m.d_twy, nullptr, 0.0); }, reps, &t);
+        if (!rc && t < best) best = t, m.sel_yf = v;
+    }
+    best = 1e30f;
+    for (int v = 0; v < m.ez->ns && !rc; ++v) {
+        rc = time_variant(c, [&] { return launch_strided(c, -1, m.ez->zs[v], m.S, m.S, nx, ny, nz, true, nshot,
+                                                         m.ncell, m.d_twz, &m, 0.0); }, reps, &t);
+        if (!rc && t < best) best = t, m.sel_z = v;
+    }
+    best = 1e30f;
+    for (int v = 0; v < m.ey->ns && !rc; ++v) {
+        rc = time_variant(c, [&] { return launch_strided(c, -1, m.ey->yi[v], m.S, m.S, nx, ny, nz, false, nshot,
+                                                         m.ncell, m.d_twy, nullptr, 0.0); }, reps, &t);
+        if (!rc && t < best) best = t, m.sel_yi = v;
+    }
+    best = 1e30f;
+    for (int v = 0; v < m.ex->nx && !rc; ++v) {
+        XArgs a = base_xargs(m, nshot);
+        a.S = m.S;
+        a.A = m.A;
+        a.Xprev = m.X[0];
+        a.Xout = m.X[1];
+        a.c1 = 0.5f;
+        rc = time_variant(c, [&] { return launch_x(c, -1, m.ex->xk[v], XM_MID, a, ny * nz, nshot, 0.0); }, reps, &t);
+        if (!rc && t < best) best = t, m.sel_x = v;
+    }
+    c->profile = prof;
+    if (!rc) m.tuned = true;
+    return rc;
 }
 
 int finalize(ose_ctx* c)
@@ -491,6 +564,10 @@ int finalize(ose_ctx* c)
             HIPCHK(c, hipMemcpy(m.d_h, h.data(), wb, hipMemcpyHostToDevice));
         }
     }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->autotune)
+        for (auto& mp : c->models)
+            if ((rc = autotune_model(c, *mp))) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->finalized = true;
     return OSE_OK;
@@ -778,6 +855,26 @@ int ose_timed_run(ose_ctx* c, int nsteps, double* ms)
     return collect(c);
 }
 
+int ose_set_autotune(ose_ctx* c, int on)
+{
+    if (!c) return OSE_EINVAL;
+    if (c->finalized) return c->fail(OSE_ESTATE, "ose_set_autotune: context already running");
+    c->autotune = on != 0;
+    return OSE_OK;
+}
+
+int ose_get_tuning(ose_ctx* c, int model_id, int* shape)
+{
+    if (!c || !shape) return OSE_EINVAL;
+    if (model_id < 0 || model_id >= (int)c->models.size()) return c->fail(OSE_EINVAL, "ose_get_tuning: bad model");
+    const Model& m = *c->models[model_id];
+    shape[0] = m.ey->yf[m.sel_yf].T;
+    shape[1] = m.ez->zs[m.sel_z].T;
+    shape[2] = m.ey->yi[m.sel_yi].T;
+    shape[3] = m.ex->xk[m.sel_x].P;
+    return OSE_OK;
+}
+
 int ose_pass_stats(ose_ctx* c, int pass, double* avg_ms, long* launches, double* bytes)
 {
     if (!c || pass < 0 || pass >= PK_N) return OSE_EINVAL;
@@ -847,7 +944,7 @@ int ose_get_wavefield(ose_ctx* c, int shot_id, float* re, float* im)
     a.A = m.A + (long)sh->slot * m.ncell;
     const bool prof = c->profile;
     c->profile = false;
-    rc = launch_x(c, 0, m.ex, XM_INV, a, m.d.nyb * m.d.nzb, 1, 0.0);
+    rc = launch_x(c, 0, m.ex->xk[m.sel_x], XM_INV, a, m.d.nyb * m.d.nzb, 1, 0.0);
     c->profile = prof;
     if (rc) return rc;
     std::vector<float2> buf((size_t)m.ncell);
@@ -880,7 +977,7 @@ int ose_set_wavefield(ose_ctx* c, int shot_id, const float* re, const float* im)
     HIPCHK(c, hipMemcpy(a.A, buf.data(), sizeof(float2) * buf.size(), hipMemcpyHostToDevice));
     const bool prof = c->profile;
     c->profile = false;
-    rc = launch_x(c, 0, m.ex, XM_FWD, a, m.d.nyb * m.d.nzb, 1, 0.0);
+    rc = launch_x(c, 0, m.ex->xk[m.sel_x], XM_FWD, a, m.d.nyb * m.d.nzb, 1, 0.0);
     c->profile = prof;
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -907,15 +1004,15 @@ int ose_apply_operator(ose_ctx* c, int model_id, const float* in, float* out)
     XArgs a = base_xargs(m, 1);
     a.A = A;
     a.Xout = X;
-    rc = launch_x(c, 0, m.ex, XM_FWD, a, ny * nz, 1, 0.0);
-    if (!rc) rc = launch_strided(c, 0, m.ey, m.ey->ystr[0], X, X, nx, ny, nz, false, 1, m.ncell, m.d_twy, nullptr, 0);
-    if (!rc) rc = launch_strided(c, 0, m.ez, m.ez->zsym, X, X, nx, ny, nz, true, 1, m.ncell, m.d_twz, &m, 0);
-    if (!rc) rc = launch_strided(c, 0, m.ey, m.ey->ystr[1], X, X, nx, ny, nz, false, 1, m.ncell, m.d_twy, nullptr, 0);
+    rc = launch_x(c, 0, m.ex->xk[m.sel_x], XM_FWD, a, ny * nz, 1, 0.0);
+    if (!rc) rc = launch_strided(c, 0, m.ey->yf[m.sel_yf], X, X, nx, ny, nz, false, 1, m.ncell, m.d_twy, nullptr, 0);
+    if (!rc) rc = launch_strided(c, 0, m.ez->zs[m.sel_z], X, X, nx, ny, nz, true, 1, m.ncell, m.d_twz, &m, 0);
+    if (!rc) rc = launch_strided(c, 0, m.ey->yi[m.sel_yi], X, X, nx, ny, nz, false, 1, m.ncell, m.d_twy, nullptr, 0);
     if (!rc) {
         XArgs b = base_xargs(m, 1);
         b.Xprev = X;
         b.A = A;
-        rc = launch_x(c, 0, m.ex, XM_INV, b, ny * nz, 1, 0.0);
+        rc = launch_x(c, 0, m.ex->xk[m.sel_x], XM_INV, b, ny * nz, 1, 0.0);
     }
     if (!rc) {
         void* args[] = {&A, &m.d_vr, &m.ncell, &m.ncell, nullptr};
@@ -974,15 +1071,15 @@ int ose_fft3d(ose_ctx* c, int nx, int ny, int nz, const float* in, float* out, i
     if (sign < 0) {
         a.A = A;
         a.Xout = X;
-        rc = launch_x(c, 0, ex, XM_FWD, a, ny * nz, 1, 0.0);
-        if (!rc) rc = launch_strided(c, 0, ey, ey->ystr[0], X, X, nx, ny, nz, false, 1, n, twy, nullptr, 0);
-        if (!rc) rc = launch_strided(c, 0, ez, ez->ystr[0], X, X, nx, ny, nz, true, 1, n, twz, nullptr, 0);
+        rc = launch_x(c, 0, ex->xk[0], XM_FWD, a, ny * nz, 1, 0.0);
+        if (!rc) rc = launch_strided(c, 0, ey->yf[0], X, X, nx, ny, nz, false, 1, n, twy, nullptr, 0);
+        if (!rc) rc = launch_strided(c, 0, ez->yf[0], X, X, nx, ny, nz, true, 1, n, twz, nullptr, 0);
     } else {
-        rc = launch_strided(c, 0, ez, ez->ystr[1], A, A, nx, ny, nz, true, 1, n, twz, nullptr, 0);
-        if (!rc) rc = launch_strided(c, 0, ey, ey->ystr[1], A, A, nx, ny, nz, false, 1, n, twy, nullptr, 0);
+        rc = launch_strided(c, 0, ez->yi[0], A, A, nx, ny, nz, true, 1, n, twz, nullptr, 0);
+        if (!rc) rc = launch_strided(c, 0, ey->yi[0], A, A, nx, ny, nz, false, 1, n, twy, nullptr, 0);
         a.Xprev = A;
         a.A = X;   // XM_INV writes IFFT_x * inv_nx (= 1 here) into A
-        if (!rc) rc = launch_x(c, 0, ex, XM_INV, a, ny * nz, 1, 0.0);
+        if (!rc) rc = launch_x(c, 0, ex->xk[0], XM_INV, a, ny * nz, 1, 0.0);
     }
     c->profile = prof;
     if (rc) return rc;

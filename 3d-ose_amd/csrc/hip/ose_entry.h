@@ -4,17 +4,30 @@
 
 namespace ose {
 
+// one compiled launch shape of a strided (y/z) pass: T x columns per workgroup
+struct StridedKern {
+    int T = 0, threads = 0;
+    size_t lds = 0;
+    const void* fn = nullptr;
+};
+
+// one compiled launch shape of the x pass: P lines per workgroup, a kernel per XMode
+struct XKern {
+    int P = 0, threads = 0;
+    size_t lds = 0;
+    const void* fn[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+};
+
+constexpr int kMaxVariants = 4;
+
 struct FftEntry {
     int N = 0, N1 = 0, N2 = 0;
-    // strided (y/z) passes: T columns per workgroup
-    int s_T = 0, s_threads = 0;
-    size_t s_lds = 0;
-    const void* ystr[2] = {nullptr, nullptr};   // [0] forward, [1] inverse
-    const void* zsym = nullptr;                 // forward-z * |k|/N * inverse-z
-    // x-line pass: P lines per workgroup, one kernel per XMode
-    int x_P = 0, x_threads = 0;
-    size_t x_lds = 0;
-    const void* xk[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    int ns = 0;                       // strided variants
+    StridedKern yf[kMaxVariants];     // forward strided transform
+    StridedKern yi[kMaxVariants];     // inverse strided transform
+    StridedKern zs[kMaxVariants];     // forward * |k|/N * inverse
+    int nx = 0;                       // x-pass variants
+    XKern xk[kMaxVariants];
 };
 
 // nullptr if N is not a compiled extent

@@ -71,6 +71,8 @@ def hip():
         _proto(L, "ose_set_options", [vp, c_int, c_int])
         _proto(L, "ose_pass_stats", [vp, c_int, dp, P(c_long), dp])
         _proto(L, "ose_reset_stats", [vp])
+        _proto(L, "ose_set_autotune", [vp, c_int])
+        _proto(L, "ose_get_tuning", [vp, c_int, ip])
         _proto(L, "ose_supported_extent", [c_int])
         _proto(L, "ose_extent_is_supported", [c_int])
         _hip = L
@@ -97,5 +99,5 @@ def hip_symbols():
         "ose_run", "ose_sync", "ose_timed_run", "ose_get_traces", "ose_get_wavefield",
         "ose_set_wavefield", "ose_get_step", "ose_set_step", "ose_apply_operator",
         "ose_fft3d", "ose_set_options", "ose_pass_stats", "ose_reset_stats",
-        "ose_supported_extent", "ose_extent_is_supported",
+        "ose_supported_extent", "ose_extent_is_supported", "ose_set_autotune", "ose_get_tuning",
     ]

@@ -88,6 +88,14 @@ class Context:
         self.shots.append(dict(model=model, nt=s.size))
         return sid.value
 
+    def set_autotune(self, on=True):
+        self._check(self._L.ose_set_autotune(self._h, int(on)))
+
+    def tuning(self, model):
+        out = (c_int * 4)()
+        self._check(self._L.ose_get_tuning(self._h, model, out))
+        return dict(zip(("T_fwd_y", "T_z", "T_inv_y", "P_x"), list(out)))
+
     def set_options(self, use_graph=True, profile=False):
         self._check(self._L.ose_set_options(self._h, int(use_graph), int(profile)))
 

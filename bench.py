@@ -83,7 +83,24 @@ def cpu_baseline(sim, info, seconds):
     }
 
 
+def rocprof_summary(db_path):
+    """Per-kernel dispatch statistics from a rocprofv3 results database."""
+    import sqlite3
+    con = sqlite3.connect(db_path)
+    rows = con.execute("select name, count(*), sum(duration), avg(duration), min(duration), max(duration) "
+                       "from kernels group by name order by sum(duration) desc").fetchall()
+    total = sum(r[2] for r in rows) or 1
+    out = [f"{'calls':>6} {'total_ms':>10} {'avg_us':>9} {'min_us':>9} {'max_us':>9} {'pct':>6}  kernel"]
+    for name, n, tot, avg, mn, mx in rows:
+        out.append(f"{n:6d} {tot / 1e6:10.3f} {avg / 1e3:9.1f} {mn / 1e3:9.1f} {mx / 1e3:9.1f} "
+                   f"{100.0 * tot / total:6.2f}  {name}")
+    return "\n".join(out)
+
+
 def main():
+    if len(sys.argv) == 3 and sys.argv[1] == "--rocprof-summary":
+        print(rocprof_summary(sys.argv[2]))
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)

@@ -122,6 +122,14 @@ int ose_set_options(ose_ctx* ctx, int use_graph, int profile);
 int ose_pass_stats(ose_ctx* ctx, int pass, double* avg_ms, long* launches, double* bytes);
 int ose_reset_stats(ose_ctx* ctx);
 
+/* Launch-shape autotuner (default on): at the first ose_run each pass of each
+ * model is timed in every compiled shape (x columns per workgroup for the
+ * strided passes, lines per workgroup for the x pass) and the fastest kept.
+ * No counterpart in the reference (cuFFT chose its own plan, ose_3d.cu:226). */
+int ose_set_autotune(ose_ctx* ctx, int on);
+/* shape[4] = {T forward-y, T z-symbol, T inverse-y, P x-pass} of a model.  */
+int ose_get_tuning(ose_ctx* ctx, int model_id, int* shape);
+
 /* Smallest compiled FFT extent >= n, or 0.                                 */
 int ose_supported_extent(int n);
 /* Nonzero if extent n is compiled.                                         */
