@@ -51,6 +51,26 @@ __global__ void k_advance(int* it)
     if (threadIdx.x == 0 && blockIdx.x == 0) *it = *it + 1;
 }
 
+// plain streaming copy, the bandwidth yardstick of the passes: each thread
+// moves four independent 16-byte vectors (64 B), one pass over the array
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__global__ void k_copy(const f32x4* __restrict__ src, f32x4* __restrict__ dst, long n)
+{
+    const long base = ((long)blockIdx.x * blockDim.x) * 4 + threadIdx.x;
+    f32x4 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const long i = base + (long)k * blockDim.x;
+        if (i < n) v[k] = __builtin_nontemporal_load(src + i);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const long i = base + (long)k * blockDim.x;
+        if (i < n) __builtin_nontemporal_store(v[k], dst + i);
+    }
+}
+
 __global__ void k_scale_vr(float2* f, const float* vr, long n, long shot_stride, float s)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -853,6 +873,75 @@ int ose_timed_run(ose_ctx* c, int nsteps, double* ms)
     (void)hipEventDestroy(e1);
     if (rc) return rc;
     return collect(c);
+}
+
+int ose_copy_probe(ose_ctx* c, size_t bytes, int reps, double* gbs)
+{
+    if (!c || !gbs || reps <= 0 || bytes < 4096) return OSE_EINVAL;
+    int rc;
+    if ((rc = set_device(c)) || (rc = ose_sync(c))) return rc;
+    DevBuf a, b;
+    bytes &= ~(size_t)15;
+    if ((rc = dev_alloc(c, a, bytes)) || (rc = dev_alloc(c, b, bytes))) return rc;
+    HIPCHK(c, hipMemsetAsync(a.p, 0, bytes, c->stream));
+    const long n = (long)(bytes / 16);
+    const dim3 grid((unsigned)((n + 1023) / 1024)), block(256);
+    const f32x4* src = (const f32x4*)a.p;
+    f32x4* dst = (f32x4*)b.p;
+    void* args[] = {(void*)&src, (void*)&dst, (void*)&n};
+    HIPCHK(c, hipLaunchKernel((const void*)&k_copy, grid, block, args, 0, c->stream));
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; ++r) HIPCHK(c, hipLaunchKernel((const void*)&k_copy, grid, block, args, 0, c->stream));
+    HIPCHK(c, hipEventRecord(e1, c->stream));
+    HIPCHK(c, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *gbs = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+    return OSE_OK;
+}
+
+int ose_pass_probe(ose_ctx* c, int model_id, int which, int reps, double* ms)
+{
+    if (!c || !ms || reps <= 0) return OSE_EINVAL;
+    if (model_id < 0 || model_id >= (int)c->models.size()) return c->fail(OSE_EINVAL, "ose_pass_probe: bad model");
+    int rc;
+    if ((rc = set_device(c)) || (rc = finalize(c)) || (rc = ose_sync(c))) return rc;
+    Model& m = *c->models[model_id];
+    const int nshot = (int)m.shots.size();
+    if (nshot == 0) return c->fail(OSE_ESTATE, "ose_pass_probe: model has no shots");
+    const int nx = m.d.nxb, ny = m.d.nyb, nz = m.d.nzb;
+    // the probe works on the spectral scratch and the second slot only, so
+    // a state being stepped is not disturbed (X1 and S are rewritten by the
+    // next step before they are read)
+    std::function<int()> body;
+    XArgs a = base_xargs(m, nshot);
+    a.S = m.S;
+    a.A = m.S;            // x probes write into scratch, not the accumulator
+    a.Xprev = m.X[1];
+    a.Xout = m.X[1];
+    a.c1 = 0.5f;
+    switch (which) {
+    case 0: body = [&] { return launch_strided(c, -1, m.ey->yf[m.sel_yf], m.X[1], m.S, nx, ny, nz, false, nshot, m.ncell, m.d_twy, nullptr, 0.0); }; break;
+    case 1: body = [&] { return launch_strided(c, -1, m.ez->yf[m.sel_z], m.S, m.S, nx, ny, nz, true, nshot, m.ncell, m.d_twz, nullptr, 0.0); }; break;
+    case 2: body = [&] { return launch_strided(c, -1, m.ez->zs[m.sel_z], m.S, m.S, nx, ny, nz, true, nshot, m.ncell, m.d_twz, &m, 0.0); }; break;
+    case 3: body = [&] { return launch_strided(c, -1, m.ey->yi[m.sel_yi], m.S, m.S, nx, ny, nz, false, nshot, m.ncell, m.d_twy, nullptr, 0.0); }; break;
+    case 4: body = [&] { return launch_x(c, -1, m.ex->xk[m.sel_x], XM_MID, a, ny * nz, nshot, 0.0); }; break;
+    case 5: body = [&] { return launch_x(c, -1, m.ex->xk[m.sel_x], XM_FWD, a, ny * nz, nshot, 0.0); }; break;
+    default: return c->fail(OSE_EINVAL, "ose_pass_probe: unknown pass %d", which);
+    }
+    float t = 0.f;
+    const bool prof = c->profile;
+    c->profile = false;
+    rc = time_variant(c, body, reps, &t);
+    c->profile = prof;
+    if (rc) return rc;
+    *ms = t;
+    return OSE_OK;
 }
 
 int ose_set_autotune(ose_ctx* c, int on)

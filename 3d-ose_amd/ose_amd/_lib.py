@@ -71,6 +71,8 @@ def hip():
         _proto(L, "ose_set_options", [vp, c_int, c_int])
         _proto(L, "ose_pass_stats", [vp, c_int, dp, P(c_long), dp])
         _proto(L, "ose_reset_stats", [vp])
+        _proto(L, "ose_copy_probe", [vp, ctypes.c_size_t, c_int, dp])
+        _proto(L, "ose_pass_probe", [vp, c_int, c_int, c_int, dp])
         _proto(L, "ose_set_autotune", [vp, c_int])
         _proto(L, "ose_get_tuning", [vp, c_int, ip])
         _proto(L, "ose_supported_extent", [c_int])
@@ -100,4 +102,5 @@ def hip_symbols():
         "ose_set_wavefield", "ose_get_step", "ose_set_step", "ose_apply_operator",
         "ose_fft3d", "ose_set_options", "ose_pass_stats", "ose_reset_stats",
         "ose_supported_extent", "ose_extent_is_supported", "ose_set_autotune", "ose_get_tuning",
+        "ose_copy_probe", "ose_pass_probe",
     ]

@@ -88,6 +88,12 @@ class Context:
         self.shots.append(dict(model=model, nt=s.size))
         return sid.value
 
+    def copy_probe(self, nbytes=1 << 30, reps=10):
+        """GB/s of a plain device copy (read + write bytes)."""
+        g = c_double()
+        self._check(self._L.ose_copy_probe(self._h, int(nbytes), int(reps), ctypes.byref(g)))
+        return g.value
+
     def set_autotune(self, on=True):
         self._check(self._L.ose_set_autotune(self._h, int(on)))
 

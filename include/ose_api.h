@@ -130,6 +130,17 @@ int ose_set_autotune(ose_ctx* ctx, int on);
 /* shape[4] = {T forward-y, T z-symbol, T inverse-y, P x-pass} of a model.  */
 int ose_get_tuning(ose_ctx* ctx, int model_id, int* shape);
 
+/* Bandwidth yardstick: a plain 16-byte-per-lane device copy of `bytes`,
+ * repeated reps times; gbs = (read + write bytes) / time.  No counterpart in
+ * the reference; it calibrates the per-pass GB/s and the HBM counters.    */
+int ose_copy_probe(ose_ctx* ctx, size_t bytes, int reps, double* gbs);
+
+/* Mean duration (ms) of one launch of a single pass over all shots of a
+ * model, on scratch buffers: which = 0 forward-y, 1 plain forward-z,
+ * 2 z-symbol, 3 inverse-y, 4 x-epilogue (middle order), 5 forward-x.
+ * Measurement utility; no counterpart in the reference.                   */
+int ose_pass_probe(ose_ctx* ctx, int model_id, int which, int reps, double* ms);
+
 /* Smallest compiled FFT extent >= n, or 0.                                 */
 int ose_supported_extent(int n);
 /* Nonzero if extent n is compiled.                                         */
