@@ -86,6 +86,7 @@ XPASS = """    {{
         k.fn[XM_LAST] = (const void*)&k_xpass<N1, N2, P, XM_LAST>;
         k.fn[XM_FWD] = (const void*)&k_xpass<N1, N2, P, XM_FWD>;
         k.fn[XM_INV] = (const void*)&k_xpass<N1, N2, P, XM_INV>;
+        k.fn[XM_LASTA] = (const void*)&k_xpass<N1, N2, P, XM_LASTA>;
         e.xk[e.nx++] = k;
     }}
 """

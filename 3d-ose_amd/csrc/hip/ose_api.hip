@@ -17,11 +17,13 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <random>
 #include <algorithm>
 #include <string>
 #include <vector>
 
 #include "../../../include/ose_api.h"
+#include "ose_bnet.h"
 #include "ose_entry.h"
 #include "ose_kernels.h"
 
@@ -156,6 +158,15 @@ struct Model {
     int* d_src_x = nullptr;
     float *d_s = nullptr, *d_h = nullptr;
     int rec_ny = 0, rec_nx = 0;
+    // learned thin pad (boundary network)
+    bool learned = false;          // a network is set for this model
+    bool learned_on = false;       // used by the step (cleared once the guard trips)
+    float guard = 2.0f;
+    BnetGeom bg{};
+    DevBuf bnw, bnbuf;
+    uint16_t* d_wbf = nullptr;     // [3][27][16][16] bf16
+    float* d_bias = nullptr;       // [3][16]
+    BnetBufs bb{};
 };
 
 struct Shot {
@@ -342,6 +353,9 @@ int launch_x(ose_ctx* c, int kind, const XKern& k, int mode, XArgs& a, int nline
     case XM_LAST:
         ok = ok && a.S && a.Xprev && a.Xout && a.A && a.vr && a.gx && a.gy && a.gz && a.traces && a.it;
         break;
+    case XM_LASTA:
+        ok = ok && a.S && a.Xprev && a.A && a.vr && a.gx && a.gy && a.gz && a.traces && a.it;
+        break;
     default: ok = false;
     }
     if (!ok) return c->fail(OSE_EINVAL, "x pass mode %d launched with a missing buffer", mode);
@@ -433,7 +447,7 @@ int enqueue_model_step(ose_ctx* c, Model& m, int* d_it)
             a.Xprev = m.X[(j - 1) % 2];
             a.c1 = (float)m.coef[j + 1];
             if (j == K - 1) {
-                mode = XM_LAST;
+                mode = m.learned_on ? XM_LASTA : XM_LAST;
                 kind = PK_XLAST;
                 a.Xout = m.X[0];
                 a.traces = m.d_traces;
@@ -454,6 +468,16 @@ int enqueue_model_step(ose_ctx* c, Model& m, int* d_it)
             }
         }
         if ((rc = launch_x(c, kind, m.ex->xk[m.sel_x], mode, a, ny * nz, nshot, bytes))) return rc;
+    }
+    if (m.learned_on) {
+        // boundary network on the pad of the new state (in A), then X0 = FFT_x(A)
+        if ((rc = bnet_forward(c->stream, m.bg, m.d_wbf, m.d_bias, m.A, m.d_vr, m.ncell, nshot, m.bb, m.guard, true)))
+            return c->fail(rc, "boundary network launch failed");
+        XArgs a = base_xargs(m, nshot);
+        a.A = m.A;
+        a.Xout = m.X[0];
+        if ((rc = launch_x(c, -1, m.ex->xk[m.sel_x], XM_FWD, a, ny * nz, nshot, 16.0 * cells)))
+            return rc;
     }
     return OSE_OK;
 }
@@ -533,6 +557,37 @@ int autotune_model(ose_ctx* c, Model& m)
     return rc;
 }
 
+// device buffers of the boundary network for nshot shots (training adds
+// the pre-activations)
+int alloc_bnet_bufs(ose_ctx* c, Model& m, int nshot, bool training)
+{
+    if (nshot > 64) return c->fail(OSE_EINVAL, "boundary network: at most 64 shots per model");
+    const long n0 = m.bg.sh[0].count, n1 = m.bg.sh[1].count, n2 = m.bg.sh[2].count, n3 = m.bg.sh[3].count;
+    size_t off = 0;
+    auto take = [&off](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
+    const size_t o_feat = take(sizeof(uint16_t) * 16 * n3 * nshot);
+    const size_t o_h1 = take(sizeof(uint16_t) * 16 * n2 * nshot);
+    const size_t o_h2 = take(sizeof(uint16_t) * 16 * n1 * nshot);
+    const size_t o_y1 = training ? take(sizeof(float) * 16 * n2) : 0;
+    const size_t o_y2 = training ? take(sizeof(float) * 16 * n1) : 0;
+    const size_t o_y3 = take(sizeof(float) * 2 * n0 * nshot);
+    const size_t o_st = take(sizeof(float) * 4 * nshot);
+    const size_t o_tr = take(sizeof(int) * 4);
+    int rc;
+    if ((rc = dev_alloc(c, m.bnbuf, off))) return rc;
+    char* b = (char*)m.bnbuf.p;
+    m.bb.feat = (uint16_t*)(b + o_feat);
+    m.bb.h1 = (uint16_t*)(b + o_h1);
+    m.bb.h2 = (uint16_t*)(b + o_h2);
+    m.bb.y1 = training ? (float*)(b + o_y1) : nullptr;
+    m.bb.y2 = training ? (float*)(b + o_y2) : nullptr;
+    m.bb.y3 = (float*)(b + o_y3);
+    m.bb.stats = (float*)(b + o_st);
+    m.bb.tripped = (int*)(b + o_tr);
+    HIPCHK(c, hipMemset(m.bb.tripped, 0, sizeof(int) * 4));
+    return OSE_OK;
+}
+
 int finalize(ose_ctx* c)
 {
     if (c->finalized) return OSE_OK;
@@ -583,6 +638,7 @@ int finalize(ose_ctx* c)
             HIPCHK(c, hipMemcpy(m.d_s, s.data(), wb, hipMemcpyHostToDevice));
             HIPCHK(c, hipMemcpy(m.d_h, h.data(), wb, hipMemcpyHostToDevice));
         }
+        if (m.learned && (rc = alloc_bnet_bufs(c, m, nshot, false))) return rc;
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (c->autotune)
@@ -819,6 +875,16 @@ int ose_run(ose_ctx* c, int nsteps)
     int rc;
     if ((rc = set_device(c))) return rc;
     if ((rc = finalize(c))) return rc;
+    // a tripped boundary-network guard switches the model back to the plain taper
+    for (auto& mp : c->models) {
+        if (!mp->learned_on) continue;
+        int tr = 0;
+        HIPCHK(c, hipMemcpy(&tr, mp->bb.tripped, sizeof(int), hipMemcpyDeviceToHost));
+        if (tr) {
+            mp->learned_on = false;
+            drop_graph(c);
+        }
+    }
     const bool graph = c->use_graph && !c->profile;
     if (graph && (rc = build_graph(c))) {
         // capture is not available: run eagerly
@@ -1174,6 +1240,325 @@ int ose_fft3d(ose_ctx* c, int nx, int ny, int nz, const float* in, float* out, i
     if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(out, X, fb, hipMemcpyDeviceToHost));
+    return OSE_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------
+// learned thin pad: set on a model, status
+// ---------------------------------------------------------------------
+static BnetGeom make_geom(int nxb, int nyb, int nzb, int bl, int bf, int bu, int nx, int ny, int nz, double vmax,
+                          double R)
+{
+    BnetGeom g{};
+    g.nxb = nxb;
+    g.nyb = nyb;
+    g.nzb = nzb;
+    g.bl = bl;
+    g.bf = bf;
+    g.bu = bu;
+    g.nx = nx;
+    g.ny = ny;
+    g.nz = nz;
+    g.vmax = (float)vmax;
+    g.R = (float)R;
+    for (int d = 0; d < 4; ++d) g.sh[d] = make_shell(nxb, nyb, nzb, bl, bf, bu, nx, ny, nz, d);
+    return g;
+}
+
+static int upload_params(ose_ctx* c, DevBuf& buf, uint16_t** wbf, uint16_t** wtbf, float** bias, const float* p,
+                         float** master)
+{
+    // layout: fp32 master [NPAR] | bf16 W [3*WSZ] | bf16 WT [3*WSZ] | fp32 bias [48]
+    const size_t o_m = 0, o_w = (sizeof(float) * BN_NPAR + 255) & ~(size_t)255;
+    const size_t o_wt = o_w + ((sizeof(uint16_t) * 3 * BN_WSZ + 255) & ~(size_t)255);
+    const size_t o_b = o_wt + ((sizeof(uint16_t) * 3 * BN_WSZ + 255) & ~(size_t)255);
+    int rc;
+    if (!buf.p && (rc = dev_alloc(c, buf, o_b + sizeof(float) * 48))) return rc;
+    char* b = (char*)buf.p;
+    float* m = (float*)(b + o_m);
+    *wbf = (uint16_t*)(b + o_w);
+    if (wtbf) *wtbf = (uint16_t*)(b + o_wt);
+    *bias = (float*)(b + o_b);
+    if (master) *master = m;
+    if (p) HIPCHK(c, hipMemcpy(m, p, sizeof(float) * BN_NPAR, hipMemcpyHostToDevice));
+    if ((rc = bnet_convert(c->stream, m, *wbf, (uint16_t*)(b + o_wt), *bias)))
+        return c->fail(rc, "boundary network weight conversion failed");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return OSE_OK;
+}
+
+extern "C" {
+
+int ose_boundary_net_size(void) { return BN_NPAR; }
+
+int ose_set_boundary_net(ose_ctx* c, int model_id, const float* params, size_t n, double guard)
+{
+    if (!c) return OSE_EINVAL;
+    if (model_id < 0 || model_id >= (int)c->models.size()) return c->fail(OSE_EINVAL, "ose_set_boundary_net: bad model");
+    Model& m = *c->models[model_id];
+    int rc;
+    if ((rc = set_device(c))) return rc;
+    if (!params) {   // remove: plain taper
+        m.learned = m.learned_on = false;
+        drop_graph(c);
+        return OSE_OK;
+    }
+    if (n != (size_t)BN_NPAR) return c->fail(OSE_EINVAL, "ose_set_boundary_net: expected %d parameters", BN_NPAR);
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(params[i]))
+            return c->fail(OSE_ENUMERIC, "ose_set_boundary_net: non-finite parameter %zu (keeping the taper)", i);
+    if (!(guard > 1.0)) return c->fail(OSE_EINVAL, "ose_set_boundary_net: guard must exceed 1");
+    const ose_model_desc& d = m.d;
+    if (d.nx < 8 || d.ny < 8 || d.nz < 8)
+        return c->fail(OSE_EINVAL, "ose_set_boundary_net: interior too small for the network's receptive field");
+    float vmax = 0.f;
+    {
+        // vmax of the model from v/R
+        std::vector<float> vr((size_t)m.ncell);
+        HIPCHK(c, hipMemcpy(vr.data(), m.d_vr, sizeof(float) * vr.size(), hipMemcpyDeviceToHost));
+        for (float x : vr) vmax = std::max(vmax, x);
+        vmax = (float)(vmax * d.R);
+    }
+    m.bg = make_geom(d.nxb, d.nyb, d.nzb, d.bl, d.bf, d.bu, d.nx, d.ny, d.nz, vmax, d.R);
+    if ((rc = upload_params(c, m.bnw, &m.d_wbf, nullptr, &m.d_bias, params, nullptr))) return rc;
+    m.guard = (float)guard;
+    m.learned = m.learned_on = true;
+    if (c->finalized && (rc = alloc_bnet_bufs(c, m, (int)m.shots.size(), false))) return rc;
+    drop_graph(c);
+    return OSE_OK;
+}
+
+int ose_boundary_status(ose_ctx* c, int model_id, int* active, int* tripped)
+{
+    if (!c) return OSE_EINVAL;
+    if (model_id < 0 || model_id >= (int)c->models.size()) return c->fail(OSE_EINVAL, "ose_boundary_status: bad model");
+    Model& m = *c->models[model_id];
+    int rc;
+    if ((rc = ose_sync(c))) return rc;
+    int tr = 0;
+    if (m.learned && m.bb.tripped) HIPCHK(c, hipMemcpy(&tr, m.bb.tripped, sizeof(int), hipMemcpyDeviceToHost));
+    if (active) *active = (m.learned && !tr) ? 1 : 0;
+    if (tripped) *tripped = tr;
+    return OSE_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------
+// training object
+// ---------------------------------------------------------------------
+struct ose_bnet {
+    ose_ctx c;                 // device, stream and error text
+    Model m;                   // geometry, v/R and network buffers
+    long ncell = 0;
+    DevBuf params, opt, scratch, fields, lossbuf;
+    float* d_master = nullptr;
+    uint16_t *d_wt = nullptr;
+    float *d_grad = nullptr, *d_m = nullptr, *d_v = nullptr;
+    uint16_t *dy_a = nullptr, *dy_b = nullptr;
+    float2 *d_u = nullptr, *d_t = nullptr;
+    std::vector<std::unique_ptr<DevBuf>> samples;   // each: u | t
+    int step = 0;
+};
+
+extern "C" {
+
+int ose_bnet_create(ose_bnet** out, int device, int nxb, int nyb, int nzb, int bl, int bf, int bu, int nx, int ny,
+                    int nz, const float* v_padded)
+{
+    if (!out || !v_padded) return OSE_EINVAL;
+    *out = nullptr;
+    if (nxb <= 0 || nyb <= 0 || nzb <= 0 || bl < 0 || bf < 0 || bu < 0 || nx < 8 || ny < 8 || nz < 8 ||
+        bl + nx > nxb || bf + ny > nyb || bu + nz > nzb)
+        return OSE_EINVAL;
+    auto* b = new (std::nothrow) ose_bnet();
+    if (!b) return OSE_ENOMEM;
+    ose_ctx* c = &b->c;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return OSE_EDEVICE;
+    }
+    const long n = (long)nxb * nyb * nzb;
+    b->ncell = n;
+    double vmax = 0.0;
+    for (long i = 0; i < n; ++i) {
+        if (!std::isfinite(v_padded[i]) || v_padded[i] <= 0.f) {
+            (void)hipStreamDestroy(c->stream);
+            delete b;
+            return OSE_ENUMERIC;
+        }
+        vmax = std::max(vmax, (double)v_padded[i]);
+    }
+    Model& m = b->m;
+    m.ncell = n;
+    m.d.nxb = nxb; m.d.nyb = nyb; m.d.nzb = nzb;
+    m.d.bl = bl; m.d.bf = bf; m.d.bu = bu; m.d.nx = nx; m.d.ny = ny; m.d.nz = nz;
+    m.d.R = vmax;   // features use v/vmax; with R = vmax, v/R is that directly
+    m.bg = make_geom(nxb, nyb, nzb, bl, bf, bu, nx, ny, nz, vmax, vmax);
+    std::vector<float> vr((size_t)n);
+    for (long i = 0; i < n; ++i) vr[i] = (float)(v_padded[i] / vmax);
+    int rc = OSE_OK;
+    if ((rc = dev_alloc(c, m.vr, sizeof(float) * n))) goto fail;
+    m.d_vr = (float*)m.vr.p;
+    if (hipMemcpy(m.d_vr, vr.data(), sizeof(float) * n, hipMemcpyHostToDevice) != hipSuccess) { rc = OSE_EDEVICE; goto fail; }
+    if ((rc = alloc_bnet_bufs(c, m, 1, true))) goto fail;
+    {
+        // grads, Adam moments
+        if ((rc = dev_alloc(c, b->opt, sizeof(float) * 3 * BN_NPAR))) goto fail;
+        b->d_grad = (float*)b->opt.p;
+        b->d_m = b->d_grad + BN_NPAR;
+        b->d_v = b->d_m + BN_NPAR;
+        if (hipMemset(b->opt.p, 0, sizeof(float) * 3 * BN_NPAR) != hipSuccess) { rc = OSE_EDEVICE; goto fail; }
+        const long n1 = m.bg.sh[1].count, n2 = m.bg.sh[2].count;
+        if ((rc = dev_alloc(c, b->scratch, sizeof(uint16_t) * 16 * (n2 + n1) + 512))) goto fail;
+        b->dy_a = (uint16_t*)b->scratch.p;
+        b->dy_b = b->dy_a + 16 * n2;
+        if ((rc = dev_alloc(c, b->fields, sizeof(float2) * 2 * n))) goto fail;
+        b->d_u = (float2*)b->fields.p;
+        b->d_t = b->d_u + n;
+        if ((rc = dev_alloc(c, b->lossbuf, sizeof(float) * (1 << 20)))) goto fail;
+        // initial parameters: He-scaled layers 1-2, zero last layer (starts as the plain taper)
+        std::vector<float> p(BN_NPAR, 0.f);
+        std::mt19937_64 rng(12345);
+        std::normal_distribution<float> nd(0.f, 1.f);
+        for (int l = 0; l < 2; ++l) {
+            const float sd = std::sqrt(2.0f / (27.0f * (l == 0 ? 9.0f : 16.0f)));
+            for (int i = 0; i < BN_WSZ; ++i) p[l * (BN_WSZ + 16) + i] = sd * nd(rng);
+        }
+        if ((rc = upload_params(c, b->params, &m.d_wbf, &b->d_wt, &m.d_bias, p.data(), &b->d_master))) goto fail;
+    }
+    *out = b;
+    return OSE_OK;
+fail:
+    (void)hipStreamDestroy(c->stream);
+    delete b;
+    return rc;
+}
+
+int ose_bnet_destroy(ose_bnet* b)
+{
+    if (!b) return OSE_EINVAL;
+    (void)hipSetDevice(b->c.device);
+    (void)hipStreamSynchronize(b->c.stream);
+    hipStream_t st = b->c.stream;
+    delete b;
+    (void)hipStreamDestroy(st);
+    return OSE_OK;
+}
+
+const char* ose_bnet_error(const ose_bnet* b) { return b ? b->c.err.c_str() : "null network"; }
+
+int ose_bnet_set_params(ose_bnet* b, const float* p, size_t n)
+{
+    if (!b || !p || n != (size_t)BN_NPAR) return OSE_EINVAL;
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(p[i])) return b->c.fail(OSE_ENUMERIC, "non-finite parameter %zu", i);
+    (void)hipSetDevice(b->c.device);
+    return upload_params(&b->c, b->params, &b->m.d_wbf, &b->d_wt, &b->m.d_bias, p, &b->d_master);
+}
+
+int ose_bnet_get_params(ose_bnet* b, float* p, size_t n)
+{
+    if (!b || !p || n != (size_t)BN_NPAR) return OSE_EINVAL;
+    ose_ctx* c = &b->c;
+    (void)hipSetDevice(c->device);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(p, b->d_master, sizeof(float) * n, hipMemcpyDeviceToHost));
+    return OSE_OK;
+}
+
+int ose_bnet_forward(ose_bnet* b, const float* u, float* delta)
+{
+    if (!b || !u || !delta) return OSE_EINVAL;
+    ose_ctx* c = &b->c;
+    (void)hipSetDevice(c->device);
+    Model& m = b->m;
+    HIPCHK(c, hipMemcpy(b->d_u, u, sizeof(float2) * b->ncell, hipMemcpyHostToDevice));
+    int rc;
+    if ((rc = bnet_forward(c->stream, m.bg, m.d_wbf, m.d_bias, b->d_u, m.d_vr, b->ncell, 1, m.bb, 0.f, false)))
+        return c->fail(rc, "bnet_forward launch failed");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const Shell& s0 = m.bg.sh[0];
+    std::vector<float> y3((size_t)s0.count * 2);
+    float st[4];
+    HIPCHK(c, hipMemcpy(y3.data(), m.bb.y3, sizeof(float) * y3.size(), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(st, m.bb.stats, sizeof(st), hipMemcpyDeviceToHost));
+    const float s = std::sqrt(st[0] / (float)m.bg.sh[3].count);
+    std::memset(delta, 0, sizeof(float) * 2 * b->ncell);
+    for (long i = 0; i < s0.count; ++i) {
+        int x, y, z;
+        shell_coords(s0, i, x, y, z);
+        const long cell = ((long)z * s0.ny + y) * s0.nx + x;
+        delta[2 * cell] = s * y3[2 * i];
+        delta[2 * cell + 1] = s * y3[2 * i + 1];
+    }
+    return OSE_OK;
+}
+
+int ose_bnet_gradient(ose_bnet* b, const float* u, const float* t, float* loss, float* grad)
+{
+    if (!b || !u || !t || !loss || !grad) return OSE_EINVAL;
+    ose_ctx* c = &b->c;
+    (void)hipSetDevice(c->device);
+    Model& m = b->m;
+    HIPCHK(c, hipMemcpy(b->d_u, u, sizeof(float2) * b->ncell, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(b->d_t, t, sizeof(float2) * b->ncell, hipMemcpyHostToDevice));
+    float* dl = (float*)b->lossbuf.p;
+    int rc;
+    if ((rc = bnet_backward(c->stream, m.bg, m.d_wbf, b->d_wt, m.d_bias, b->d_u, b->d_t, m.d_vr, b->ncell, m.bb,
+                            b->dy_a, b->dy_b, b->d_grad, dl)))
+        return c->fail(rc, "bnet_backward launch failed");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(loss, dl, sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(grad, b->d_grad, sizeof(float) * BN_NPAR, hipMemcpyDeviceToHost));
+    return OSE_OK;
+}
+
+int ose_bnet_add_sample(ose_bnet* b, const float* u, const float* t, int* count)
+{
+    if (!b || !u || !t) return OSE_EINVAL;
+    ose_ctx* c = &b->c;
+    (void)hipSetDevice(c->device);
+    auto s = std::make_unique<DevBuf>();
+    int rc;
+    if ((rc = dev_alloc(c, *s, sizeof(float2) * 2 * b->ncell))) return rc;
+    HIPCHK(c, hipMemcpy(s->p, u, sizeof(float2) * b->ncell, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy((float2*)s->p + b->ncell, t, sizeof(float2) * b->ncell, hipMemcpyHostToDevice));
+    b->samples.push_back(std::move(s));
+    if (count) *count = (int)b->samples.size();
+    return OSE_OK;
+}
+
+int ose_bnet_train(ose_bnet* b, int iters, float lr, unsigned long long seed, float* losses)
+{
+    if (!b || iters <= 0 || iters > (1 << 20) || !(lr > 0.f)) return OSE_EINVAL;
+    ose_ctx* c = &b->c;
+    if (b->samples.empty()) return c->fail(OSE_ESTATE, "ose_bnet_train: no samples");
+    (void)hipSetDevice(c->device);
+    Model& m = b->m;
+    std::mt19937_64 rng(seed);
+    float* dl = (float*)b->lossbuf.p;
+    const uint16_t* wt_dummy = nullptr;
+    (void)wt_dummy;
+    for (int it = 0; it < iters; ++it) {
+        const DevBuf& smp = *b->samples[rng() % b->samples.size()];
+        const float2* u = (const float2*)smp.p;
+        const float2* t = u + b->ncell;
+        int rc;
+        if ((rc = bnet_backward(c->stream, m.bg, m.d_wbf, b->d_wt, m.d_bias, u, t, m.d_vr, b->ncell, m.bb, b->dy_a,
+                                b->dy_b, b->d_grad, dl + it)))
+            return c->fail(rc, "bnet_backward launch failed");
+        ++b->step;
+        if ((rc = bnet_adam(c->stream, b->d_master, b->d_grad, b->d_m, b->d_v, b->step, lr, 0.9f, 0.999f, 1e-8f)))
+            return c->fail(rc, "adam launch failed");
+        if ((rc = bnet_convert(c->stream, b->d_master, m.d_wbf, b->d_wt, m.d_bias)))
+            return c->fail(rc, "weight conversion failed");
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (losses) HIPCHK(c, hipMemcpy(losses, dl, sizeof(float) * iters, hipMemcpyDeviceToHost));
     return OSE_OK;
 }
 

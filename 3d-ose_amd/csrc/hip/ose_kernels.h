@@ -171,6 +171,7 @@ enum XMode : int {
     XM_LAST = 2,     // as MID, u = g (A + c W') is the new state: receivers, out X0
     XM_FWD = 3,      // out = FFT_x(A)              (state upload)
     XM_INV = 4,      // A = IFFT_x(Xprev) / nx      (state download)
+    XM_LASTA = 5,    // as LAST, but the new state u goes to A (physical) for the boundary network
 };
 
 struct XArgs {
@@ -304,7 +305,7 @@ void k_xpass(XArgs a)
         const float gyz = a.gy[y] * a.gz[z];
         int itv = 0;
         bool rec = false;
-        if (MODE == XM_LAST) {
+        if (MODE == XM_LAST || MODE == XM_LASTA) {
             itv = *a.it;
             rec = (z == a.rec_z) && (y >= a.rec_y0) && (y < a.rec_y0 + a.rec_ny) && (itv < a.nt_cap);
         }
@@ -325,9 +326,10 @@ void k_xpass(XArgs a)
                 const float g = gyz * a.gx[x];
                 const float2 acc = a.A[lbase + x];
                 const float2 an = make_float2(g * (acc.x + a.c1 * wn.x), g * (acc.y + a.c1 * wn.y));
-                if (MODE == XM_MID) {
+                if (MODE == XM_MID || MODE == XM_LASTA) {
                     a.A[lbase + x] = an;
-                } else {
+                }
+                if (MODE == XM_LAST || MODE == XM_LASTA) {
                     wn = an;   // the new state u; its x transform is X_0 of the next step
                     if (rec && x >= a.rec_x0 && x < a.rec_x0 + a.rec_nx)
                         a.traces[(long)blockIdx.y * a.trace_shot_stride +
@@ -337,7 +339,7 @@ void k_xpass(XArgs a)
             u[n1] = wn;
         }
     }
-    x_forward<N1, N2, C::PN>(u, okl, t, L, a.tw, w, a.Xout + lbase);
+    if (MODE != XM_LASTA) x_forward<N1, N2, C::PN>(u, okl, t, L, a.tw, w, a.Xout + lbase);
 }
 
 // ---------------------------------------------------------------------

@@ -102,6 +102,19 @@ class Context:
         self._check(self._L.ose_get_tuning(self._h, model, out))
         return dict(zip(("T_fwd_y", "T_z", "T_inv_y", "P_x"), list(out)))
 
+    def set_boundary_net(self, model, params, guard=2.0):
+        """Enable the learned thin pad on a model (None removes it)."""
+        if params is None:
+            self._check(self._L.ose_set_boundary_net(self._h, model, None, 0, float(guard)))
+            return
+        p = _f32(params)
+        self._check(self._L.ose_set_boundary_net(self._h, model, _ptr(p), p.size, float(guard)))
+
+    def boundary_status(self, model):
+        a, t = c_int(), c_int()
+        self._check(self._L.ose_boundary_status(self._h, model, ctypes.byref(a), ctypes.byref(t)))
+        return dict(active=bool(a.value), tripped=bool(t.value))
+
     def set_options(self, use_graph=True, profile=False):
         self._check(self._L.ose_set_options(self._h, int(use_graph), int(profile)))
 
@@ -183,3 +196,79 @@ def supported_extent(n):
 
 def extent_is_supported(n):
     return bool(_lib.hip().ose_extent_is_supported(int(n)))
+
+
+def boundary_net_size():
+    return _lib.hip().ose_boundary_net_size()
+
+
+class BoundaryNet:
+    """Training object of the learned thin-pad network (ose_bnet_*)."""
+
+    def __init__(self, v_padded, origin, interior, device=0):
+        self._L = _lib.hip()
+        v = _f32(v_padded)
+        nzb, nyb, nxb = v.shape
+        h = ctypes.c_void_p()
+        rc = self._L.ose_bnet_create(ctypes.byref(h), device, nxb, nyb, nzb, origin[0], origin[1], origin[2],
+                                     interior[0], interior[1], interior[2], _ptr(v))
+        if rc:
+            raise OseError(rc, "ose_bnet_create failed")
+        self._h = h
+        self.shape = v.shape
+        self.n = boundary_net_size()
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.ose_bnet_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc:
+            raise OseError(rc, self._L.ose_bnet_error(self._h).decode())
+
+    @staticmethod
+    def _c(a):
+        return np.ascontiguousarray(a, dtype=np.complex64)
+
+    def params(self):
+        p = np.empty(self.n, dtype=np.float32)
+        self._check(self._L.ose_bnet_get_params(self._h, _ptr(p), self.n))
+        return p
+
+    def set_params(self, p):
+        p = _f32(p)
+        self._check(self._L.ose_bnet_set_params(self._h, _ptr(p), p.size))
+
+    def forward(self, u):
+        u = self._c(u)
+        d = np.empty_like(u)
+        self._check(self._L.ose_bnet_forward(self._h, u.view(np.float32).ctypes.data_as(_fp),
+                                             d.view(np.float32).ctypes.data_as(_fp)))
+        return d
+
+    def gradient(self, u, t):
+        u, t = self._c(u), self._c(t)
+        loss = ctypes.c_float()
+        g = np.empty(self.n, dtype=np.float32)
+        self._check(self._L.ose_bnet_gradient(self._h, u.view(np.float32).ctypes.data_as(_fp),
+                                              t.view(np.float32).ctypes.data_as(_fp), ctypes.byref(loss), _ptr(g)))
+        return loss.value, g
+
+    def add_sample(self, u, t):
+        u, t = self._c(u), self._c(t)
+        cnt = c_int()
+        self._check(self._L.ose_bnet_add_sample(self._h, u.view(np.float32).ctypes.data_as(_fp),
+                                                t.view(np.float32).ctypes.data_as(_fp), ctypes.byref(cnt)))
+        return cnt.value
+
+    def train(self, iters, lr=1e-3, seed=0):
+        losses = np.empty(iters, dtype=np.float32)
+        self._check(self._L.ose_bnet_train(self._h, int(iters), float(lr), int(seed), _ptr(losses)))
+        return losses

@@ -141,6 +141,44 @@ int ose_copy_probe(ose_ctx* ctx, size_t bytes, int reps, double* gbs);
  * Measurement utility; no counterpart in the reference.                   */
 int ose_pass_probe(ose_ctx* ctx, int model_id, int which, int reps, double* ms);
 
+/* ---- learned thin pad ------------------------------------------------
+ * A 3-layer 3D convolution (9 input features -> 16 -> 16 -> 2 channels,
+ * 3x3x3 taps, GELU; BF16 weights, FP32 accumulation on MFMA) that corrects
+ * the pad cells of the new state after every step: u' = u + s * net(u/s, v,
+ * distance into the pad), s the RMS of u on the network's receptive shell.
+ * It replaces the reference's wide absorbing layer (generate_att /
+ * apply_ABC_Xt, ose_3d_kernel.cu:242-288) with a thin one; the classical
+ * taper of the model stays active underneath.  Parameters: ose_boundary_net_size()
+ * floats, [W1[27][16][16] (tap, out, in), b1[16], W2, b2, W3, b3].          */
+int ose_boundary_net_size(void);
+/* Enable the network on a model (params == NULL removes it).  guard > 1:
+ * if one step would multiply the pad energy by more than guard (or produce
+ * a non-finite value), the correction is not applied and the model falls
+ * back to the plain taper for the rest of the run.  Non-finite parameters
+ * are rejected with OSE_ENUMERIC (the taper stays).                        */
+int ose_set_boundary_net(ose_ctx* ctx, int model_id, const float* params, size_t n, double guard);
+/* active: the network is in use; tripped: its guard has fired.             */
+int ose_boundary_status(ose_ctx* ctx, int model_id, int* active, int* tripped);
+
+/* Training object of the network on one padded grid (single sample at a
+ * time; BF16 forward and backward on MFMA, FP32 Adam).  Fields u, t are
+ * interleaved complex [nzb][nyb][nxb]; v_padded in m/s.                   */
+typedef struct ose_bnet ose_bnet;
+int ose_bnet_create(ose_bnet** net, int device, int nxb, int nyb, int nzb, int bl, int bf, int bu,
+                    int nx, int ny, int nz, const float* v_padded);
+int ose_bnet_destroy(ose_bnet* net);
+const char* ose_bnet_error(const ose_bnet* net);
+int ose_bnet_set_params(ose_bnet* net, const float* params, size_t n);
+int ose_bnet_get_params(ose_bnet* net, float* params, size_t n);
+/* delta = s * net(...) on the pad cells, zero elsewhere (interleaved).    */
+int ose_bnet_forward(ose_bnet* net, const float* u, float* delta);
+/* loss = mean over pad cells of |u/s + y - t/s|^2 and its gradient.      */
+int ose_bnet_gradient(ose_bnet* net, const float* u, const float* t, float* loss, float* grad);
+/* Keep a training pair on the device.                                     */
+int ose_bnet_add_sample(ose_bnet* net, const float* u, const float* t, int* count);
+/* iters Adam steps on randomly drawn stored pairs; losses[iters].          */
+int ose_bnet_train(ose_bnet* net, int iters, float lr, unsigned long long seed, float* losses);
+
 /* Smallest compiled FFT extent >= n, or 0.                                 */
 int ose_supported_extent(int n);
 /* Nonzero if extent n is compiled.                                         */
