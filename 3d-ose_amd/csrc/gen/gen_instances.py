@@ -25,7 +25,7 @@ SIZES = {
     16: (4, 4), 24: (4, 6), 32: (4, 8), 48: (6, 8), 64: (8, 8), 80: (8, 10),
     96: (8, 12), 112: (8, 14), 128: (8, 16), 140: (10, 14), 144: (12, 12),
     152: (8, 19), 160: (10, 16), 192: (12, 16), 200: (10, 20), 224: (14, 16),
-    240: (15, 16), 256: (16, 16), 280: (14, 20), 288: (16, 18), 320: (16, 20),
+    240: (15, 16), 256: (16, 16), 272: (16, 17), 280: (14, 20), 288: (16, 18), 320: (16, 20),
     336: (16, 21), 378: (18, 21), 384: (16, 24), 400: (20, 20), 448: (16, 28),
     480: (20, 24), 512: (16, 32), 576: (24, 24), 600: (20, 30), 640: (20, 32),
     672: (21, 32), 720: (24, 30), 768: (24, 32), 784: (28, 28), 800: (25, 32),

@@ -161,7 +161,10 @@ struct Model {
     // learned thin pad (boundary network)
     bool learned = false;          // a network is set for this model
     bool learned_on = false;       // used by the step (cleared once the guard trips)
+    bool replace_taper = false;    // while the network is on, the pad taper is 1
     float guard = 2.0f;
+    DevBuf ones;                   // nxb+nyb+nzb: the taper of the learned mode (1, or the
+                                   // classical taper once the guard trips)
     BnetGeom bg{};
     DevBuf bnw, bnbuf;
     uint16_t* d_wbf = nullptr;     // [3][27][16][16] bf16
@@ -384,6 +387,11 @@ XArgs base_xargs(const Model& m, int nshot)
     a.gx = m.d_gx;
     a.gy = m.d_gy;
     a.gz = m.d_gz;
+    if (m.learned_on && m.replace_taper && m.ones.p) {
+        a.gx = (const float*)m.ones.p;
+        a.gy = a.gx + m.d.nxb;
+        a.gz = a.gy + m.d.nyb;
+    }
     a.tw = m.d_twx;
     a.shot_stride = m.ncell;
     a.ny = m.d.nyb;
@@ -473,6 +481,12 @@ int enqueue_model_step(ose_ctx* c, Model& m, int* d_it)
         // boundary network on the pad of the new state (in A), then X0 = FFT_x(A)
         if ((rc = bnet_forward(c->stream, m.bg, m.d_wbf, m.d_bias, m.A, m.d_vr, m.ncell, nshot, m.bb, m.guard, true)))
             return c->fail(rc, "boundary network launch failed");
+        if (m.replace_taper) {
+            float* eff = (float*)m.ones.p;
+            if ((rc = bnet_taper_select(c->stream, eff, m.d_gx, m.d.nxb, m.d_gy, m.d.nyb, m.d_gz, m.d.nzb,
+                                        m.bb.tripped)))
+                return c->fail(rc, "boundary network taper launch failed");
+        }
         XArgs a = base_xargs(m, nshot);
         a.A = m.A;
         a.Xout = m.X[0];
@@ -1293,7 +1307,7 @@ extern "C" {
 
 int ose_boundary_net_size(void) { return BN_NPAR; }
 
-int ose_set_boundary_net(ose_ctx* c, int model_id, const float* params, size_t n, double guard)
+int ose_set_boundary_net(ose_ctx* c, int model_id, const float* params, size_t n, double guard, int replace_taper)
 {
     if (!c) return OSE_EINVAL;
     if (model_id < 0 || model_id >= (int)c->models.size()) return c->fail(OSE_EINVAL, "ose_set_boundary_net: bad model");
@@ -1324,6 +1338,13 @@ int ose_set_boundary_net(ose_ctx* c, int model_id, const float* params, size_t n
     m.bg = make_geom(d.nxb, d.nyb, d.nzb, d.bl, d.bf, d.bu, d.nx, d.ny, d.nz, vmax, d.R);
     if ((rc = upload_params(c, m.bnw, &m.d_wbf, nullptr, &m.d_bias, params, nullptr))) return rc;
     m.guard = (float)guard;
+    m.replace_taper = replace_taper != 0;
+    if (m.replace_taper) {
+        const int nt = d.nxb + d.nyb + d.nzb;
+        if (!m.ones.p && (rc = dev_alloc(c, m.ones, sizeof(float) * nt))) return rc;
+        std::vector<float> one((size_t)nt, 1.f);
+        HIPCHK(c, hipMemcpy(m.ones.p, one.data(), sizeof(float) * nt, hipMemcpyHostToDevice));
+    }
     m.learned = m.learned_on = true;
     if (c->finalized && (rc = alloc_bnet_bufs(c, m, (int)m.shots.size(), false))) return rc;
     drop_graph(c);
@@ -1486,14 +1507,22 @@ int ose_bnet_forward(ose_bnet* b, const float* u, float* delta)
     float st[4];
     HIPCHK(c, hipMemcpy(y3.data(), m.bb.y3, sizeof(float) * y3.size(), hipMemcpyDeviceToHost));
     HIPCHK(c, hipMemcpy(st, m.bb.stats, sizeof(st), hipMemcpyDeviceToHost));
-    const float s = std::sqrt(st[0] / (float)m.bg.sh[3].count);
+    (void)st;
     std::memset(delta, 0, sizeof(float) * 2 * b->ncell);
     for (long i = 0; i < s0.count; ++i) {
         int x, y, z;
         shell_coords(s0, i, x, y, z);
         const long cell = ((long)z * s0.ny + y) * s0.nx + x;
-        delta[2 * cell] = s * y3[2 * i];
-        delta[2 * cell + 1] = s * y3[2 * i + 1];
+        const float ur = u[2 * cell], ui = u[2 * cell + 1];
+        float qx = 1.f + y3[2 * i], qy = y3[2 * i + 1];   // passive gain G = q / max(1, |q|)
+        const float m2 = qx * qx + qy * qy;
+        if (m2 > 1.f) {
+            const float r = 1.f / std::sqrt(m2);
+            qx *= r;
+            qy *= r;
+        }
+        delta[2 * cell] = ur * (qx - 1.f) - ui * qy;     // u (G - 1) on the pad
+        delta[2 * cell + 1] = ui * (qx - 1.f) + ur * qy;
     }
     return OSE_OK;
 }

@@ -2,8 +2,9 @@
 //
 // A 3-layer 3D convolution (9 input features padded to 16 channels -> 16 ->
 // 16 -> 2, 3^3 taps, GELU) applied after every OSE step to the pad of a thin
-// absorbing layer: u' = u + s * net(u / s, v, geometry) on the pad cells,
-// with s the RMS of u over the network's receptive shell.  Activations live
+// absorbing layer: a learned passive complex gain u' = u G, G = q/max(1,|q|),
+// q = 1 + y, y = net(u / s, v, geometry) on the pad cells, s the RMS of u
+// over the receptive shell.  Activations live
 // only on the shell R_d = {cells within distance d of the pad} (d = 3..0),
 // stored compactly as three boxes on the periodic grid (a z band, a y band
 // of the remaining z, an x band of the remaining z and y).  Every GEMM -
@@ -96,11 +97,15 @@ struct BnetBufs {
 };
 
 // Launch the forward pass for all shots of a field (u in physical space,
-// full padded grid, shot stride ncell) and add s*y to the pad cells of u
-// unless the energy guard trips (then u is left as is and *tripped = 1).
+// full padded grid, shot stride ncell) and multiply the pad cells of u by
+// G(y) unless the energy guard trips (then u is left as is, *tripped = 1).
 // Wbf: bf16 weights [3][27][16][16]; bias: fp32 [3][16].
 int bnet_forward(hipStream_t st, const BnetGeom& g, const uint16_t* Wbf, const float* bias,
                  float2* u, const float* vr, long ncell, int nshot, BnetBufs& b, float guard, bool apply);
+
+// eff[0:nx+ny+nz] = 1 while *tripped == 0, else the classical tapers gx|gy|gz
+int bnet_taper_select(hipStream_t st, float* eff, const float* gx, int nx, const float* gy, int ny, const float* gz,
+                      int nz, const int* tripped);
 
 // Training: one sample, forward + backward into grad (fp32 [BN_NPAR], zeroed
 // here), loss (device scalar).  t: target field, same layout as u.

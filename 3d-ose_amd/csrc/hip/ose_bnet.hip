@@ -206,24 +206,41 @@ __global__ __launch_bounds__(256) void k_bn_conv(ConvArgs a)
 }
 
 // ---------------------------------------------------------------------
-// guard + apply: u' = u + s y on the pad unless the pad energy grows by more
-// than `guard` or turns non-finite; the flag is sticky.
-// stats[shot*4 + 1] += sum |u + s y|^2, stats[shot*4 + 2] += sum |u|^2 (R_0)
-// ---------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_bn_guard(const float2* u, long ncell, Shell s0, Shell s3,
-                                                  const float* y3, long y_shot, float* stats)
+// guard + apply: u' = u G(y) on the pad unless the pad energy grows by more
+// than `guard` or turns non-finite; the flag is sticky.  (|G| <= 1, so the
+// guard only catches non-finite outputs.)
+// stats[shot*4 + 1] += sum |u G|^2, stats[shot*4 + 2] += sum |u|^2 (R_0)
+// passive gain G = q / max(1, |q|), q = 1 + a + i b, applied to v
+__device__ __forceinline__ float2 gain_q(float a, float b)
+{
+    float qx = 1.f + a, qy = b;
+    const float m2 = qx * qx + qy * qy;
+    if (m2 > 1.f) {
+        const float r = rsqrtf(m2);
+        qx *= r;
+        qy *= r;
+    }
+    return make_float2(qx, qy);
+}
+
+__device__ __forceinline__ float2 pad_gain(float2 v, float a, float b)
+{
+    const float2 g = gain_q(a, b);
+    return make_float2(v.x * g.x - v.y * g.y, v.y * g.x + v.x * g.y);
+}
+
+__global__ __launch_bounds__(256) void k_bn_guard(const float2* u, long ncell, Shell s0, const float* y3, long y_shot,
+                                                  float* stats)
 {
     __shared__ float red[8];
     const int shot = blockIdx.y;
-    const float s = sqrtf(stats[shot * 4 + 0] / (float)s3.count);
     float acc[2] = {0.f, 0.f};
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < s0.count; i += (long)gridDim.x * blockDim.x) {
         int x, y, z;
         shell_coords(s0, i, x, y, z);
         const float2 v = u[(long)shot * ncell + ((long)z * s0.ny + y) * s0.nx + x];
-        const float nx = v.x + s * y3[(long)shot * y_shot + i * 2];
-        const float ny = v.y + s * y3[(long)shot * y_shot + i * 2 + 1];
-        acc[0] += nx * nx + ny * ny;
+        const float2 w = pad_gain(v, y3[(long)shot * y_shot + i * 2], y3[(long)shot * y_shot + i * 2 + 1]);
+        acc[0] += w.x * w.x + w.y * w.y;
         acc[1] += v.x * v.x + v.y * v.y;
     }
     block_sum<2>(acc, red);
@@ -233,35 +250,48 @@ __global__ __launch_bounds__(256) void k_bn_guard(const float2* u, long ncell, S
     }
 }
 
-__global__ __launch_bounds__(256) void k_bn_apply(float2* u, long ncell, Shell s0, Shell s3, const float* y3,
-                                                  long y_shot, const float* stats, int nshot, float guard,
-                                                  int* tripped)
+__global__ __launch_bounds__(256) void k_bn_apply(float2* u, long ncell, Shell s0, const float* y3, long y_shot,
+                                                  const float* stats, int nshot, float guard, int* tripped)
 {
     // every block evaluates the same guard decision
     bool bad = *tripped != 0;
     for (int k = 0; k < nshot && !bad; ++k) {
-        const float post = stats[k * 4 + 1], pre = stats[k * 4 + 2];
-        if (!(post <= guard * pre + 1e-30f)) bad = true;
+        // energy growth of the pad beyond `guard`, ignoring pads that hold
+        // less than 1e-4 of the receptive shell's energy (leakage noise)
+        const float post = stats[k * 4 + 1], pre = stats[k * 4 + 2], shell = stats[k * 4 + 0];
+        if (!(post <= guard * pre + 1e-4f * shell + 1e-30f)) bad = true;
     }
     if (bad) {
         if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *tripped = 1;
         return;
     }
     const int shot = blockIdx.y;
-    const float s = sqrtf(stats[shot * 4 + 0] / (float)s3.count);
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < s0.count; i += (long)gridDim.x * blockDim.x) {
         int x, y, z;
         shell_coords(s0, i, x, y, z);
         float2* p = u + (long)shot * ncell + ((long)z * s0.ny + y) * s0.nx + x;
-        const float2 v = *p;
-        *p = make_float2(v.x + s * y3[(long)shot * y_shot + i * 2], v.y + s * y3[(long)shot * y_shot + i * 2 + 1]);
+        *p = pad_gain(*p, y3[(long)shot * y_shot + i * 2], y3[(long)shot * y_shot + i * 2 + 1]);
     }
+}
+
+// taper of a learned-mode step: 1 while the network is on, the classical
+// taper once the guard has tripped (device-side, so the switch takes effect
+// from the next step even inside one graph replay)
+__global__ __launch_bounds__(256) void k_bn_taper(float* eff, const float* gx, int nx, const float* gy, int ny,
+                                                  const float* gz, int nz, const int* tripped)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool tr = *tripped != 0;
+    if (i < nx) eff[i] = tr ? gx[i] : 1.f;
+    else if (i < nx + ny) eff[i] = tr ? gy[i - nx] : 1.f;
+    else if (i < nx + ny + nz) eff[i] = tr ? gz[i - nx - ny] : 1.f;
 }
 
 // ---------------------------------------------------------------------
 // training kernels
 // ---------------------------------------------------------------------
-// loss on the pad: e = u/s + y - t/s; dy3 = 2 e / M (bf16, channels 0, 1)
+// loss on the pad: e = (u/s) G(y) - t/s; for y = a + ib
+// dL/da = 2 Re(conj(e) (u/s) dG/da) / M, likewise b (bf16)
 __global__ __launch_bounds__(256) void k_bn_loss(const float2* u, const float2* t, Shell s0, Shell s3,
                                                  const float* y3, const float* stats, uint16_t* dy3, float* loss)
 {
@@ -274,13 +304,27 @@ __global__ __launch_bounds__(256) void k_bn_loss(const float2* u, const float2* 
         int x, y, z;
         shell_coords(s0, i, x, y, z);
         const long cell = ((long)z * s0.ny + y) * s0.nx + x;
-        const float2 uv = u[cell], tv = t[cell];
-        const float ex = (uv.x - tv.x) * inv + y3[i * 2];
-        const float ey = (uv.y - tv.y) * inv + y3[i * 2 + 1];
+        const float2 un = make_float2(u[cell].x * inv, u[cell].y * inv);
+        const float2 tn = make_float2(t[cell].x * inv, t[cell].y * inv);
+        const float a = y3[i * 2], b = y3[i * 2 + 1];
+        const float2 w = pad_gain(un, a, b);
+        const float ex = w.x - tn.x, ey = w.y - tn.y;
         acc[0] += ex * ex + ey * ey;
+        // dG/da, dG/db: (1, i) inside the unit disc; on the clipped branch
+        // G = q/|q|: dG/da = 1/|q| - q Re(q)/|q|^3, dG/db = i/|q| - q Im(q)/|q|^3
+        const float qx = 1.f + a, qy = b, m2 = qx * qx + qy * qy;
+        float2 ga = make_float2(1.f, 0.f), gb = make_float2(0.f, 1.f);
+        if (m2 > 1.f) {
+            const float r = rsqrtf(m2), r3 = r * r * r;
+            ga = make_float2(r - qx * qx * r3, -qy * qx * r3);
+            gb = make_float2(-qx * qy * r3, r - qy * qy * r3);
+        }
+        // Re(conj(e) u_n dG), e = (ex, ey)
+        const float2 ua = make_float2(un.x * ga.x - un.y * ga.y, un.x * ga.y + un.y * ga.x);
+        const float2 ub = make_float2(un.x * gb.x - un.y * gb.y, un.x * gb.y + un.y * gb.x);
         uint16_t* o = dy3 + i * 16;
-        o[0] = f2bf(scale * ex);
-        o[1] = f2bf(scale * ey);
+        o[0] = f2bf(scale * (ex * ua.x + ey * ua.y));
+        o[1] = f2bf(scale * (ex * ub.x + ey * ub.y));
 #pragma unroll
         for (int k = 2; k < 16; ++k) o[k] = 0;
     }
@@ -509,16 +553,24 @@ int bnet_forward(hipStream_t st, const BnetGeom& g, const uint16_t* Wbf, const f
     if (apply) {
         long ys = s0.count * 2;
         {
-            void* args[] = {&u, &ncell, (void*)&s0, (void*)&s3, &b.y3, &ys, &b.stats};
+            void* args[] = {&u, &ncell, (void*)&s0, &b.y3, &ys, &b.stats};
             LCHK(hipLaunchKernel((const void*)&k_bn_guard, dim3(blocks_for(s0.count, 256, 1024), nshot), dim3(256),
                                  args, 0, st));
         }
         {
-            void* args[] = {&u, &ncell, (void*)&s0, (void*)&s3, &b.y3, &ys, &b.stats, &nshot, &guard, &b.tripped};
+            void* args[] = {&u, &ncell, (void*)&s0, &b.y3, &ys, &b.stats, &nshot, &guard, &b.tripped};
             LCHK(hipLaunchKernel((const void*)&k_bn_apply, dim3(blocks_for(s0.count, 256, 1024), nshot), dim3(256),
                                  args, 0, st));
         }
     }
+    return OSE_OK;
+}
+
+int bnet_taper_select(hipStream_t st, float* eff, const float* gx, int nx, const float* gy, int ny, const float* gz,
+                      int nz, const int* tripped)
+{
+    void* args[] = {&eff, &gx, &nx, &gy, &ny, &gz, &nz, &tripped};
+    LCHK(hipLaunchKernel((const void*)&k_bn_taper, dim3((nx + ny + nz + 255) / 256), dim3(256), args, 0, st));
     return OSE_OK;
 }
 

@@ -76,7 +76,7 @@ def hip():
         _proto(L, "ose_set_autotune", [vp, c_int])
         _proto(L, "ose_get_tuning", [vp, c_int, ip])
         _proto(L, "ose_boundary_net_size", [])
-        _proto(L, "ose_set_boundary_net", [vp, c_int, fp, ctypes.c_size_t, c_double])
+        _proto(L, "ose_set_boundary_net", [vp, c_int, fp, ctypes.c_size_t, c_double, c_int])
         _proto(L, "ose_boundary_status", [vp, c_int, ip, ip])
         _proto(L, "ose_bnet_create", [P(vp), c_int] + [c_int] * 9 + [fp])
         _proto(L, "ose_bnet_destroy", [vp])

@@ -102,13 +102,14 @@ class Context:
         self._check(self._L.ose_get_tuning(self._h, model, out))
         return dict(zip(("T_fwd_y", "T_z", "T_inv_y", "P_x"), list(out)))
 
-    def set_boundary_net(self, model, params, guard=2.0):
+    def set_boundary_net(self, model, params, guard=2.0, replace_taper=True):
         """Enable the learned thin pad on a model (None removes it)."""
         if params is None:
-            self._check(self._L.ose_set_boundary_net(self._h, model, None, 0, float(guard)))
+            self._check(self._L.ose_set_boundary_net(self._h, model, None, 0, float(guard), 0))
             return
         p = _f32(params)
-        self._check(self._L.ose_set_boundary_net(self._h, model, _ptr(p), p.size, float(guard)))
+        self._check(self._L.ose_set_boundary_net(self._h, model, _ptr(p), p.size, float(guard),
+                                                 int(replace_taper)))
 
     def boundary_status(self, model):
         a, t = c_int(), c_int()

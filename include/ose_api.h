@@ -144,7 +144,8 @@ int ose_pass_probe(ose_ctx* ctx, int model_id, int which, int reps, double* ms);
 /* ---- learned thin pad ------------------------------------------------
  * A 3-layer 3D convolution (9 input features -> 16 -> 16 -> 2 channels,
  * 3x3x3 taps, GELU; BF16 weights, FP32 accumulation on MFMA) that corrects
- * the pad cells of the new state after every step: u' = u + s * net(u/s, v,
+ * the pad cells of the new state after every step with a learned passive
+ * complex gain: u' = u G, G = q / max(1, |q|), q = 1 + y, y = net(u/s, v,
  * distance into the pad), s the RMS of u on the network's receptive shell.
  * It replaces the reference's wide absorbing layer (generate_att /
  * apply_ABC_Xt, ose_3d_kernel.cu:242-288) with a thin one; the classical
@@ -154,9 +155,12 @@ int ose_boundary_net_size(void);
 /* Enable the network on a model (params == NULL removes it).  guard > 1:
  * if one step would multiply the pad energy by more than guard (or produce
  * a non-finite value), the correction is not applied and the model falls
- * back to the plain taper for the rest of the run.  Non-finite parameters
+ * back to the plain taper for the rest of the run.  replace_taper != 0: while
+ * the network is on, the model's taper is 1 (the network is the absorbing
+ * layer); the fallback restores the classical taper.  Non-finite parameters
  * are rejected with OSE_ENUMERIC (the taper stays).                        */
-int ose_set_boundary_net(ose_ctx* ctx, int model_id, const float* params, size_t n, double guard);
+int ose_set_boundary_net(ose_ctx* ctx, int model_id, const float* params, size_t n, double guard,
+                         int replace_taper);
 /* active: the network is in use; tripped: its guard has fired.             */
 int ose_boundary_status(ose_ctx* ctx, int model_id, int* active, int* tripped);
 
@@ -170,9 +174,9 @@ int ose_bnet_destroy(ose_bnet* net);
 const char* ose_bnet_error(const ose_bnet* net);
 int ose_bnet_set_params(ose_bnet* net, const float* params, size_t n);
 int ose_bnet_get_params(ose_bnet* net, float* params, size_t n);
-/* delta = s * net(...) on the pad cells, zero elsewhere (interleaved).    */
+/* delta = u (G - 1) on the pad cells, zero elsewhere (interleaved).      */
 int ose_bnet_forward(ose_bnet* net, const float* u, float* delta);
-/* loss = mean over pad cells of |u/s + y - t/s|^2 and its gradient.      */
+/* loss = mean over pad cells of |(u/s) G - t/s|^2 and its gradient.     */
 int ose_bnet_gradient(ose_bnet* net, const float* u, const float* t, float* loss, float* grad);
 /* Keep a training pair on the device.                                     */
 int ose_bnet_add_sample(ose_bnet* net, const float* u, const float* t, int* count);

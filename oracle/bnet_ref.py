@@ -18,12 +18,15 @@ interior box.  R_d is the set of cells within Chebyshev distance d of P
   h1 = gelu(conv(f,  W1) + b1)   on R_2
   h2 = gelu(conv(h1, W2) + b2)   on R_1
   y  =      conv(h2, W3) + b3    on R_0, channels 0, 1
-  u'[c] = u[c] + s (y0[c] + i y1[c])   for c in P
+  u'[c] = u[c] G(y[c]),  G(y) = q / max(1, |q|),  q = 1 + y0 + i y1,  c in P
+The correction is a learned passive complex gain on the pad (|G| <= 1, so
+re-applying it every step cannot grow energy): it vanishes where u does and
+is invariant to the scale of u (the features see u/s).
 
 conv(h, W)[c, o] = sum_{tap} sum_i h[c + off(tap), i] W[tap, o, i] with
 tap = (dz+1)*9 + (dy+1)*3 + (dx+1), offsets taken modulo the grid.
 gelu(x) = x/2 (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))).
-Training loss: L = mean_{c in P} |u[c]/s + y[c] - t[c]/s|^2.
+Training loss: L = mean_{c in P} |(u[c]/s) G(y[c]) - t[c]/s|^2.
 
 Parameters, flat float32 (the C-ABI order): W1[27][16][16], b1[16],
 W2[27][16][16], b2[16], W3[27][16][16], b3[16].
@@ -128,6 +131,18 @@ def conv_wgrad(h, g):
     return dW
 
 
+def gain(y0, y1):
+    """G and its partial derivatives dG/dy0, dG/dy1 (complex arrays)."""
+    q = 1.0 + y0 + 1j * y1
+    a = np.abs(q)
+    big = a > 1.0
+    G = np.where(big, q / np.where(big, a, 1.0), q)
+    a3 = np.where(big, a ** 3, 1.0)
+    ga = np.where(big, 1.0 / np.where(big, a, 1.0) - q * q.real / a3, 1.0 + 0j)
+    gb = np.where(big, 1j / np.where(big, a, 1.0) - q * q.imag / a3, 1j)
+    return G, ga, gb
+
+
 def features(geo, u, v, vmax):
     r3 = geo.R[3]
     s = np.sqrt(np.mean(np.abs(u[r3]) ** 2))
@@ -151,7 +166,8 @@ def forward(geo, params, u, v, vmax):
     y2 = (conv(h1, W2) + b2) * geo.R[1][..., None]
     h2 = gelu(y2) * geo.R[1][..., None]
     y3 = (conv(h2, W3) + b3) * geo.R[0][..., None]
-    delta = s * (y3[..., 0] + 1j * y3[..., 1])
+    G, _, _ = gain(y3[..., 0], y3[..., 1])
+    delta = np.where(geo.R[0], u * (G - 1.0), 0.0)
     return delta, dict(f=f, s=s, y1=y1, h1=h1, y2=y2, h2=h2, y3=y3)
 
 
@@ -161,12 +177,14 @@ def loss_and_grad(geo, params, u, v, vmax, t):
     s = c["s"]
     P = geo.R[0]
     M = P.sum()
-    e = np.zeros(u.shape + (C,))
-    er = u / s + c["y3"][..., 0] + 1j * c["y3"][..., 1] - t / s
-    e[..., 0] = np.where(P, er.real, 0.0)
-    e[..., 1] = np.where(P, er.imag, 0.0)
-    L = float(np.sum(e[..., 0] ** 2 + e[..., 1] ** 2) / M)
-    g3 = 2.0 * e / M
+    un = u / s
+    G, ga, gb = gain(c["y3"][..., 0], c["y3"][..., 1])
+    er = np.where(P, un * G - t / s, 0.0)
+    L = float(np.sum(np.abs(er) ** 2) / M)
+    # dL/dy_k = 2 Re(conj(e) u_n dG/dy_k) / M
+    g3 = np.zeros(u.shape + (C,))
+    g3[..., 0] = 2.0 * np.real(np.conj(er) * un * ga) / M
+    g3[..., 1] = 2.0 * np.real(np.conj(er) * un * gb) / M
     dW3 = conv_wgrad(c["h2"], g3)
     db3 = g3.sum(axis=(0, 1, 2))
     g2 = conv_T(g3, W3) * geo.R[1][..., None] * gelu_grad(c["y2"])

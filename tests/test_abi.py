@@ -40,7 +40,8 @@ def test_host_only_entry_points():
     assert L.ose_extent_is_supported(152) and L.ose_extent_is_supported(864)
     assert not L.ose_extent_is_supported(17)
     assert L.ose_supported_extent(848) == 864
-    assert L.ose_supported_extent(272) == 280
+    assert L.ose_supported_extent(272) == 272
+    assert L.ose_supported_extent(273) == 280
     assert L.ose_supported_extent(141) == 144
     assert L.ose_add_model(None, None, None, None) == _lib.OSE_EINVAL
     assert L.ose_run(None, 1) == _lib.OSE_EINVAL

@@ -75,11 +75,11 @@ def test_bnet_training_reduces_loss():
     assert losses[-20:].mean() < 0.5 * losses[:5].mean()
 
 
-def _learned_case():
-    return Case((36, 20, 36), (6, 6, 6, 6, 6, 6), layered(36, 20, 36, seed=9))
+def _learned_case(taper="ref"):
+    return Case((36, 20, 36), (6, 6, 6, 6, 6, 6), layered(36, 20, 36, seed=9), taper=taper)
 
 
-def _run(case, params, nt=60, guard=2.0):
+def _run(case, params, nt=60, guard=2.0, replace_taper=True, chunks=1):
     import ose_amd
     s = ricker(nt, 30.0, case.dt, delay=10)
     h = hilbert_unnorm(s)
@@ -87,23 +87,31 @@ def _run(case, params, nt=60, guard=2.0):
         m = case.add_to(c)
         sid = c.add_shot(m, (case.bl + 18, case.bf + 10, case.bu + 2), s, h)
         if params is not None:
-            c.set_boundary_net(m, params, guard)
-        c.run(nt)
+            c.set_boundary_net(m, params, guard, replace_taper=replace_taper)
+        for k in range(chunks):
+            c.run(nt // chunks)
         tr = c.traces(sid)
         re, im = c.wavefield(sid)
         st = c.boundary_status(m) if params is not None else None
     return tr, re + 1j * im, st
 
 
-def test_zero_last_layer_equals_plain_taper():
+def test_zero_last_layer_is_identity_gain():
+    """G = 1: on top of the taper the run is the plain taper; replacing the
+    taper it is the untapered run."""
     print("learned pad with zero last layer", flush=True)
-    case = _learned_case()
     p = (0.1 * np.random.default_rng(4).standard_normal(B.NPAR)).astype(np.float32)
     p[-(B.NTAP * 256 + 16):] = 0.0
+    case = _learned_case()
     tr0, u0, _ = _run(case, None)
-    tr1, u1, st = _run(case, p)
+    tr1, u1, st = _run(case, p, replace_taper=False)
     assert st["active"] and not st["tripped"]
     assert rel(tr1, tr0) < 1e-6 and rel(u1, u0) < 1e-6
+    tr2, u2, _ = _run(_learned_case("none"), None)
+    tr3, u3, st = _run(case, p, replace_taper=True)
+    assert st["active"] and not st["tripped"]
+    assert rel(tr3, tr2) < 1e-6 and rel(u3, u2) < 1e-6
+    assert rel(u3, u0) > 1e-4            # the taper does act on this case
 
 
 def test_learned_pad_changes_only_through_the_pad():
@@ -117,15 +125,48 @@ def test_learned_pad_changes_only_through_the_pad():
     assert rel(u1, u0) > 1e-6          # the correction acts
 
 
-def test_guard_falls_back_to_taper():
-    print("learned pad guard", flush=True)
+@pytest.mark.parametrize("replace_taper,chunks", [(False, 1), (True, 1), (True, 3)])
+def test_guard_falls_back_to_taper(replace_taper, chunks):
+    """A finite network whose output overflows (non-finite gain) trips the
+    guard on its first step; from then on the run is the plain taper, also
+    within one ose_run (graph replays) when the network replaced the taper.
+    In that mode the tripping step itself ran untapered, so the reference is
+    one untapered step followed by the plain taper."""
+    print(f"learned pad guard replace={replace_taper} chunks={chunks}", flush=True)
     case = _learned_case()
-    p = np.zeros(B.NPAR, dtype=np.float32)
-    p[-16] = 1e4                        # b3[0]: a huge constant correction on the pad
-    tr0, u0, _ = _run(case, None)
-    tr1, u1, st = _run(case, p, guard=2.0)
+    p = (0.1 * np.random.default_rng(6).standard_normal(B.NPAR)).astype(np.float32)
+    w = B.NTAP * 256 + 16
+    p[w:w + B.NTAP * 256] = 1e30        # W2
+    p[2 * w:2 * w + B.NTAP * 256] = 1e30  # W3
+    tr0, u0, _ = _run_switched(case) if replace_taper else _run(case, None)
+    tr1, u1, st = _run(case, p, guard=2.0, replace_taper=replace_taper, chunks=chunks)
     assert st["tripped"] and not st["active"]
     assert rel(tr1, tr0) < 1e-6 and rel(u1, u0) < 1e-6
+
+
+def _run_switched(case, nt=60):
+    """One untapered step, then the plain taper (the run a guard trip on the
+    first step of a taper-replacing network produces)."""
+    import ose_amd
+    s = ricker(nt, 30.0, case.dt, delay=10)
+    h = hilbert_unnorm(s)
+    src = (case.bl + 18, case.bf + 10, case.bu + 2)
+    with ose_amd.Context(0) as c0:
+        m0 = _learned_case("none").add_to(c0)
+        s0 = c0.add_shot(m0, src, s, h)
+        c0.run(1)
+        re, im = c0.wavefield(s0)
+        tr_first = c0.traces(s0)[:, :1].copy()
+    with ose_amd.Context(0) as c:
+        m = case.add_to(c)
+        sid = c.add_shot(m, src, s, h)
+        c.set_wavefield(sid, re, im)
+        c.step = 1
+        c.run(nt - 1)
+        tr = c.traces(sid)
+        tr[:, :1] = tr_first
+        r2, i2 = c.wavefield(sid)
+    return tr, r2 + 1j * i2, None
 
 
 def test_nonfinite_parameters_rejected():

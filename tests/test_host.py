@@ -138,7 +138,7 @@ def test_snapshot_order(H, tmp_path):
     ("preset=ref", (152, 152, 152), 9),
     ("preset=c0", (140, 140, 140), None),
     ("preset=c1", (336, 336, 336), None),
-    ("preset=c3", (280, 280, 280), None),
+    ("preset=c3", (272, 272, 272), None),
 ])
 def test_presets_build(H, preset, padded, K):
     sim = H.Simulation(preset).build()
