@@ -37,21 +37,35 @@ LDS_MAX = 160 * 1024
 
 
 def strided_variants(n1, n2):
+    """(T, split) launch shapes: T x columns per workgroup; split = 1
+    exchanges real and imaginary parts separately (half the LDS).  The split
+    form is compiled where the whole-complex exchange needs more than a
+    quarter of a CU's LDS (it then doubles the workgroups per CU)."""
     nt = max(n1, n2)
     out = []
     for t in STRIDED_TS:
-        if t * nt <= 1024 and 8 * n1 * n2 * (t + 1) <= LDS_MAX:
-            out.append(t)
+        if t * nt > 1024:
+            continue
+        full = 8 * n1 * n2 * (t + 1)
+        if full <= LDS_MAX:
+            out.append((t, 0))
+        if full > LDS_MAX // 4 and full // 2 <= LDS_MAX:
+            out.append((t, 1))
     return out
 
 
 def x_variants(n1, n2):
+    """(P, split) launch shapes: P lines per workgroup; the split (half-LDS)
+    exchange where a line's whole-complex exchange exceeds 2 KiB."""
     nt = max(n1, n2)
     pn = n1 | 1
     out = []
     for p in sorted({max(1, 128 // nt), max(1, 256 // nt)}):
-        if p * nt <= 1024 and 8 * p * n2 * pn <= LDS_MAX:
-            out.append(p)
+        full = 8 * p * n2 * pn
+        if p * nt <= 1024 and full <= LDS_MAX:
+            out.append((p, 0))
+            if 8 * n2 * pn > 2048:
+                out.append((p, 1))
     return out
 
 
@@ -66,27 +80,27 @@ void fill_entry_{N}(FftEntry& e)
     e.N = {N}; e.N1 = N1; e.N2 = N2;
 """
 STRIDED = """    {{
-        constexpr int T = {T};
-        using SC = StridedCfg<N1, N2, T>;
+        constexpr int T = {T}, SPLIT = {SPLIT};
+        using SC = StridedCfg<N1, N2, T, SPLIT>;
         StridedKern k;
-        k.T = T; k.threads = SC::THREADS; k.lds = SC::LDS_BYTES;
-        k.fn = (const void*)&k_strided<N1, N2, T, -1>; e.yf[e.ns] = k;
-        k.fn = (const void*)&k_strided<N1, N2, T, 1>; e.yi[e.ns] = k;
-        k.fn = (const void*)&k_zsym<N1, N2, T>; e.zs[e.ns] = k;
+        k.T = T; k.split = SPLIT; k.threads = SC::THREADS; k.lds = SC::LDS_BYTES;
+        k.fn = (const void*)&k_strided<N1, N2, T, -1, SPLIT>; e.yf[e.ns] = k;
+        k.fn = (const void*)&k_strided<N1, N2, T, 1, SPLIT>; e.yi[e.ns] = k;
+        k.fn = (const void*)&k_zsym<N1, N2, T, SPLIT>; e.zs[e.ns] = k;
         ++e.ns;
     }}
 """
 XPASS = """    {{
-        constexpr int P = {P};
-        using XC = XCfg<N1, N2, P>;
+        constexpr int P = {P}, SPLIT = {SPLIT};
+        using XC = XCfg<N1, N2, P, SPLIT>;
         XKern k;
-        k.P = P; k.threads = XC::THREADS; k.lds = XC::LDS_BYTES;
-        k.fn[XM_ORDER0] = (const void*)&k_xpass<N1, N2, P, XM_ORDER0>;
-        k.fn[XM_MID] = (const void*)&k_xpass<N1, N2, P, XM_MID>;
-        k.fn[XM_LAST] = (const void*)&k_xpass<N1, N2, P, XM_LAST>;
-        k.fn[XM_FWD] = (const void*)&k_xpass<N1, N2, P, XM_FWD>;
-        k.fn[XM_INV] = (const void*)&k_xpass<N1, N2, P, XM_INV>;
-        k.fn[XM_LASTA] = (const void*)&k_xpass<N1, N2, P, XM_LASTA>;
+        k.P = P; k.split = SPLIT; k.threads = XC::THREADS; k.lds = XC::LDS_BYTES;
+        k.fn[XM_ORDER0] = (const void*)&k_xpass<N1, N2, P, XM_ORDER0, SPLIT>;
+        k.fn[XM_MID] = (const void*)&k_xpass<N1, N2, P, XM_MID, SPLIT>;
+        k.fn[XM_LAST] = (const void*)&k_xpass<N1, N2, P, XM_LAST, SPLIT>;
+        k.fn[XM_FWD] = (const void*)&k_xpass<N1, N2, P, XM_FWD, SPLIT>;
+        k.fn[XM_INV] = (const void*)&k_xpass<N1, N2, P, XM_INV, SPLIT>;
+        k.fn[XM_LASTA] = (const void*)&k_xpass<N1, N2, P, XM_LASTA, SPLIT>;
         e.xk[e.nx++] = k;
     }}
 """
@@ -97,10 +111,10 @@ TAIL = """}}
 
 def instance_source(n, n1, n2):
     src = HEAD.format(N=n, N1=n1, N2=n2)
-    for t in strided_variants(n1, n2):
-        src += STRIDED.format(T=t)
-    for p in x_variants(n1, n2):
-        src += XPASS.format(P=p)
+    for t, split in strided_variants(n1, n2):
+        src += STRIDED.format(T=t, SPLIT=split)
+    for p, split in x_variants(n1, n2):
+        src += XPASS.format(P=p, SPLIT=split)
     return src + TAIL.format()
 
 

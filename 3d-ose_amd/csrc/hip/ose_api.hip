@@ -46,6 +46,11 @@ __global__ void k_inject(InjectArgs a)
     float2* p = a.X0 + (long)shot * a.shot_stride + a.src_line[shot] + k;
     const float2 o = *p;
     *p = make_float2(o.x + s * e.x - h * e.y, o.y + s * e.y + h * e.x);
+    if (k == 0) {
+        float2* q = a.W0 + (long)shot * a.shot_stride + a.src_line[shot] + sx;
+        const float2 w = *q;
+        *q = make_float2(w.x + s, w.y + h);
+    }
 }
 
 __global__ void k_advance(int* it)
@@ -152,7 +157,9 @@ struct Model {
     std::vector<int> shots;        // context shot ids
     int nt_cap = 0;
     DevBuf fields, traces, src, wav;
-    float2 *X[2] = {nullptr, nullptr}, *S = nullptr, *A = nullptr;
+    // X: x-spectral work field (passes 1-3 in place), W[0] the physical state
+    // u = W_0 and with W[1] the alternating physical W_j, A the accumulator
+    float2 *X = nullptr, *W[2] = {nullptr, nullptr}, *A = nullptr;
     float* d_traces = nullptr;
     long* d_src_line = nullptr;
     int* d_src_x = nullptr;
@@ -351,13 +358,13 @@ int launch_x(ose_ctx* c, int kind, const XKern& k, int mode, XArgs& a, int nline
     switch (mode) {
     case XM_FWD: ok = ok && a.A && a.Xout; break;
     case XM_INV: ok = ok && a.A && a.Xprev; break;
-    case XM_ORDER0: ok = ok && a.S && a.Xprev && a.Xout && a.A && a.vr; break;
-    case XM_MID: ok = ok && a.S && a.Xprev && a.Xout && a.A && a.vr && a.gx && a.gy && a.gz; break;
+    case XM_ORDER0: ok = ok && a.S && a.Win && a.Wout && a.Xout && a.A && a.vr; break;
+    case XM_MID: ok = ok && a.S && a.Win && a.Wout && a.Xout && a.A && a.vr && a.gx && a.gy && a.gz; break;
     case XM_LAST:
-        ok = ok && a.S && a.Xprev && a.Xout && a.A && a.vr && a.gx && a.gy && a.gz && a.traces && a.it;
+        ok = ok && a.S && a.Win && a.Wout && a.Xout && a.A && a.vr && a.gx && a.gy && a.gz && a.traces && a.it;
         break;
     case XM_LASTA:
-        ok = ok && a.S && a.Xprev && a.A && a.vr && a.gx && a.gy && a.gz && a.traces && a.it;
+        ok = ok && a.S && a.Win && a.Wout && a.A && a.vr && a.gx && a.gy && a.gz && a.traces && a.it;
         break;
     default: ok = false;
     }
@@ -409,10 +416,11 @@ int enqueue_model_step(ose_ctx* c, Model& m, int* d_it)
     const int nx = m.d.nxb, ny = m.d.nyb, nz = m.d.nzb;
     const long cells = m.ncell * nshot;
     int rc;
-    // source injection into X0 (x-spectral)
+    // source injection into X0 (x-spectral) and W_0 (physical)
     {
         InjectArgs a{};
-        a.X0 = m.X[0];
+        a.X0 = m.X;
+        a.W0 = m.W[0];
         a.shot_stride = m.ncell;
         a.nx = nx;
         a.src_line = m.d_src_line;
@@ -428,36 +436,38 @@ int enqueue_model_step(ose_ctx* c, Model& m, int* d_it)
     }
     const int K = m.d.K;
     for (int j = 0; j < K; ++j) {
-        float2* Xj = m.X[j % 2];
-        if ((rc = launch_strided(c, PK_YF, m.ey->yf[m.sel_yf], Xj, m.S, nx, ny, nz, false, nshot,
+        // passes 1-3 in place on X_j
+        if ((rc = launch_strided(c, PK_YF, m.ey->yf[m.sel_yf], m.X, m.X, nx, ny, nz, false, nshot,
                                  m.ncell, m.d_twy, nullptr, 16.0 * cells)))
             return rc;
-        if ((rc = launch_strided(c, PK_Z, m.ez->zs[m.sel_z], m.S, m.S, nx, ny, nz, true, nshot,
+        if ((rc = launch_strided(c, PK_Z, m.ez->zs[m.sel_z], m.X, m.X, nx, ny, nz, true, nshot,
                                  m.ncell, m.d_twz, &m, 16.0 * cells)))
             return rc;
-        if ((rc = launch_strided(c, PK_YI, m.ey->yi[m.sel_yi], m.S, m.S, nx, ny, nz, false, nshot,
+        if ((rc = launch_strided(c, PK_YI, m.ey->yi[m.sel_yi], m.X, m.X, nx, ny, nz, false, nshot,
                                  m.ncell, m.d_twy, nullptr, 16.0 * cells)))
             return rc;
         XArgs a = base_xargs(m, nshot);
-        a.S = m.S;
+        a.S = m.X;
         a.A = m.A;
+        a.Xout = m.X;
         int mode, kind;
         double bytes;
         if (j == 0) {
             mode = XM_ORDER0;
             kind = PK_X0;
-            a.Xprev = m.X[0];
-            a.Xout = m.X[1];
+            a.Win = m.W[0];
+            a.Wout = m.W[1];
             a.c0 = (float)m.coef[0];
             a.c1 = (float)m.coef[1];
-            bytes = 36.0 * cells;   // S, X0, vr read; X1, A written
+            bytes = 44.0 * cells;   // S, W0, vr read; W1, X1, A written
         } else {
-            a.Xprev = m.X[(j - 1) % 2];
+            // W_{j-1} lives in W[(j-1)%2] and W_{j+1} replaces it
+            a.Win = a.Wout = m.W[(j - 1) % 2];
             a.c1 = (float)m.coef[j + 1];
             if (j == K - 1) {
                 mode = m.learned_on ? XM_LASTA : XM_LAST;
                 kind = PK_XLAST;
-                a.Xout = m.X[0];
+                a.Wout = m.W[0];    // the new state u
                 a.traces = m.d_traces;
                 a.trace_shot_stride = (long)m.nt_cap * m.rec_ny * m.rec_nx;
                 a.rec_z = m.d.bu + m.d.rec_z;
@@ -467,19 +477,20 @@ int enqueue_model_step(ose_ctx* c, Model& m, int* d_it)
                 a.rec_nx = m.rec_nx;
                 a.it = d_it;
                 a.nt_cap = m.nt_cap;
-                bytes = 36.0 * cells;   // S, Xprev, vr, A read; X0 written
+                // S, W_{K-2}, vr, A read; u and (not LASTA) X0 written
+                bytes = (m.learned_on ? 36.0 : 44.0) * cells;
             } else {
                 mode = XM_MID;
                 kind = PK_XMID;
-                a.Xout = m.X[(j + 1) % 2];
-                bytes = 44.0 * cells;   // S, Xprev, vr, A read; X', A written
+                bytes = 52.0 * cells;   // S, W_{j-1}, vr, A read; W_{j+1}, X_{j+1}, A written
             }
         }
         if ((rc = launch_x(c, kind, m.ex->xk[m.sel_x], mode, a, ny * nz, nshot, bytes))) return rc;
     }
     if (m.learned_on) {
-        // boundary network on the pad of the new state (in A), then X0 = FFT_x(A)
-        if ((rc = bnet_forward(c->stream, m.bg, m.d_wbf, m.d_bias, m.A, m.d_vr, m.ncell, nshot, m.bb, m.guard, true)))
+        // boundary network on the pad of the new state u (in W[0]), then X0 = FFT_x(u)
+        if ((rc = bnet_forward(c->stream, m.bg, m.d_wbf, m.d_bias, m.W[0], m.d_vr, m.ncell, nshot, m.bb, m.guard,
+                               true)))
             return c->fail(rc, "boundary network launch failed");
         if (m.replace_taper) {
             float* eff = (float*)m.ones.p;
@@ -488,8 +499,8 @@ int enqueue_model_step(ose_ctx* c, Model& m, int* d_it)
                 return c->fail(rc, "boundary network taper launch failed");
         }
         XArgs a = base_xargs(m, nshot);
-        a.A = m.A;
-        a.Xout = m.X[0];
+        a.A = m.W[0];
+        a.Xout = m.X;
         if ((rc = launch_x(c, -1, m.ex->xk[m.sel_x], XM_FWD, a, ny * nz, nshot, 16.0 * cells)))
             return rc;
     }
@@ -539,29 +550,29 @@ int autotune_model(ose_ctx* c, Model& m)
     const int reps = std::max(1, std::min(5, (int)(2e8 / ((double)m.ncell * nshot))));
     best = 1e30f;
     for (int v = 0; v < m.ey->ns && !rc; ++v) {
-        rc = time_variant(c, [&] { return launch_strided(c, -1, m.ey->yf[v], m.X[0], m.S, nx, ny, nz, false, nshot,
+        rc = time_variant(c, [&] { return launch_strided(c, -1, m.ey->yf[v], m.X, m.X, nx, ny, nz, false, nshot,
                                                          m.ncell, m.d_twy, nullptr, 0.0); }, reps, &t);
         if (!rc && t < best) best = t, m.sel_yf = v;
     }
     best = 1e30f;
     for (int v = 0; v < m.ez->ns && !rc; ++v) {
-        rc = time_variant(c, [&] { return launch_strided(c, -1, m.ez->zs[v], m.S, m.S, nx, ny, nz, true, nshot,
+        rc = time_variant(c, [&] { return launch_strided(c, -1, m.ez->zs[v], m.X, m.X, nx, ny, nz, true, nshot,
                                                          m.ncell, m.d_twz, &m, 0.0); }, reps, &t);
         if (!rc && t < best) best = t, m.sel_z = v;
     }
     best = 1e30f;
     for (int v = 0; v < m.ey->ns && !rc; ++v) {
-        rc = time_variant(c, [&] { return launch_strided(c, -1, m.ey->yi[v], m.S, m.S, nx, ny, nz, false, nshot,
+        rc = time_variant(c, [&] { return launch_strided(c, -1, m.ey->yi[v], m.X, m.X, nx, ny, nz, false, nshot,
                                                          m.ncell, m.d_twy, nullptr, 0.0); }, reps, &t);
         if (!rc && t < best) best = t, m.sel_yi = v;
     }
     best = 1e30f;
     for (int v = 0; v < m.ex->nx && !rc; ++v) {
         XArgs a = base_xargs(m, nshot);
-        a.S = m.S;
+        a.S = m.X;
         a.A = m.A;
-        a.Xprev = m.X[0];
-        a.Xout = m.X[1];
+        a.Win = a.Wout = m.W[1];
+        a.Xout = m.X;
         a.c1 = 0.5f;
         rc = time_variant(c, [&] { return launch_x(c, -1, m.ex->xk[v], XM_MID, a, ny * nz, nshot, 0.0); }, reps, &t);
         if (!rc && t < best) best = t, m.sel_x = v;
@@ -618,9 +629,9 @@ int finalize(ose_ctx* c)
         const size_t fbytes = sizeof(float2) * (size_t)m.ncell * nshot;
         if ((rc = dev_alloc(c, m.fields, 4 * fbytes))) return rc;
         char* base = (char*)m.fields.p;
-        m.X[0] = (float2*)base;
-        m.X[1] = (float2*)(base + fbytes);
-        m.S = (float2*)(base + 2 * fbytes);
+        m.X = (float2*)base;
+        m.W[0] = (float2*)(base + fbytes);
+        m.W[1] = (float2*)(base + 2 * fbytes);
         m.A = (float2*)(base + 3 * fbytes);
         HIPCHK(c, hipMemsetAsync(m.fields.p, 0, 4 * fbytes, c->stream));
         const size_t tbytes = sizeof(float) * (size_t)m.nt_cap * m.rec_ny * m.rec_nx * nshot;
@@ -995,21 +1006,23 @@ int ose_pass_probe(ose_ctx* c, int model_id, int which, int reps, double* ms)
     const int nshot = (int)m.shots.size();
     if (nshot == 0) return c->fail(OSE_ESTATE, "ose_pass_probe: model has no shots");
     const int nx = m.d.nxb, ny = m.d.nyb, nz = m.d.nzb;
-    // the probe works on the spectral scratch and the second slot only, so
-    // a state being stepped is not disturbed (X1 and S are rewritten by the
-    // next step before they are read)
+    // the probe works on W[1] and A only, which hold nothing between steps
+    // (ORDER0 writes both before reading them), so a state being stepped is
+    // not disturbed; the x probe keeps its spectrum and W in W[1] (same
+    // traffic, different lines of the buffer are never shared across threads)
     std::function<int()> body;
     XArgs a = base_xargs(m, nshot);
-    a.S = m.S;
-    a.A = m.S;            // x probes write into scratch, not the accumulator
-    a.Xprev = m.X[1];
-    a.Xout = m.X[1];
+    a.S = m.W[1];
+    a.Xout = m.W[1];
+    a.A = m.A;
+    a.Win = a.Wout = m.W[1];
     a.c1 = 0.5f;
+    float2* F = m.W[1];
     switch (which) {
-    case 0: body = [&] { return launch_strided(c, -1, m.ey->yf[m.sel_yf], m.X[1], m.S, nx, ny, nz, false, nshot, m.ncell, m.d_twy, nullptr, 0.0); }; break;
-    case 1: body = [&] { return launch_strided(c, -1, m.ez->yf[m.sel_z], m.S, m.S, nx, ny, nz, true, nshot, m.ncell, m.d_twz, nullptr, 0.0); }; break;
-    case 2: body = [&] { return launch_strided(c, -1, m.ez->zs[m.sel_z], m.S, m.S, nx, ny, nz, true, nshot, m.ncell, m.d_twz, &m, 0.0); }; break;
-    case 3: body = [&] { return launch_strided(c, -1, m.ey->yi[m.sel_yi], m.S, m.S, nx, ny, nz, false, nshot, m.ncell, m.d_twy, nullptr, 0.0); }; break;
+    case 0: body = [&] { return launch_strided(c, -1, m.ey->yf[m.sel_yf], F, F, nx, ny, nz, false, nshot, m.ncell, m.d_twy, nullptr, 0.0); }; break;
+    case 1: body = [&] { return launch_strided(c, -1, m.ez->yf[m.sel_z], F, F, nx, ny, nz, true, nshot, m.ncell, m.d_twz, nullptr, 0.0); }; break;
+    case 2: body = [&] { return launch_strided(c, -1, m.ez->zs[m.sel_z], F, F, nx, ny, nz, true, nshot, m.ncell, m.d_twz, &m, 0.0); }; break;
+    case 3: body = [&] { return launch_strided(c, -1, m.ey->yi[m.sel_yi], F, F, nx, ny, nz, false, nshot, m.ncell, m.d_twy, nullptr, 0.0); }; break;
     case 4: body = [&] { return launch_x(c, -1, m.ex->xk[m.sel_x], XM_MID, a, ny * nz, nshot, 0.0); }; break;
     case 5: body = [&] { return launch_x(c, -1, m.ex->xk[m.sel_x], XM_FWD, a, ny * nz, nshot, 0.0); }; break;
     default: return c->fail(OSE_EINVAL, "ose_pass_probe: unknown pass %d", which);
@@ -1037,10 +1050,68 @@ int ose_get_tuning(ose_ctx* c, int model_id, int* shape)
     if (!c || !shape) return OSE_EINVAL;
     if (model_id < 0 || model_id >= (int)c->models.size()) return c->fail(OSE_EINVAL, "ose_get_tuning: bad model");
     const Model& m = *c->models[model_id];
-    shape[0] = m.ey->yf[m.sel_yf].T;
-    shape[1] = m.ez->zs[m.sel_z].T;
-    shape[2] = m.ey->yi[m.sel_yi].T;
-    shape[3] = m.ex->xk[m.sel_x].P;
+    auto code = [](const StridedKern& k) { return k.split ? -k.T : k.T; };
+    shape[0] = code(m.ey->yf[m.sel_yf]);
+    shape[1] = code(m.ez->zs[m.sel_z]);
+    shape[2] = code(m.ey->yi[m.sel_yi]);
+    shape[3] = m.ex->xk[m.sel_x].split ? -m.ex->xk[m.sel_x].P : m.ex->xk[m.sel_x].P;
+    return OSE_OK;
+}
+
+int ose_set_tuning(ose_ctx* c, int model_id, const int* shape)
+{
+    if (!c || !shape) return OSE_EINVAL;
+    if (model_id < 0 || model_id >= (int)c->models.size()) return c->fail(OSE_EINVAL, "ose_set_tuning: bad model");
+    Model& m = *c->models[model_id];
+    auto code = [](const StridedKern& k) { return k.split ? -k.T : k.T; };
+    auto find_strided = [&](const FftEntry* e, int which, int want) {
+        for (int v = 0; v < e->ns; ++v) {
+            const StridedKern& k = which == 0 ? e->yf[v] : which == 1 ? e->zs[v] : e->yi[v];
+            if (code(k) == want) return v;
+        }
+        return -1;
+    };
+    const int yf = find_strided(m.ey, 0, shape[0]);
+    const int z = find_strided(m.ez, 1, shape[1]);
+    const int yi = find_strided(m.ey, 2, shape[2]);
+    int x = -1;
+    for (int v = 0; v < m.ex->nx; ++v)
+        if ((m.ex->xk[v].split ? -m.ex->xk[v].P : m.ex->xk[v].P) == shape[3]) x = v;
+    if (yf < 0 || z < 0 || yi < 0 || x < 0)
+        return c->fail(OSE_EINVAL, "ose_set_tuning: launch shape {%d, %d, %d, %d} is not compiled for %dx%dx%d",
+                       shape[0], shape[1], shape[2], shape[3], m.d.nxb, m.d.nyb, m.d.nzb);
+    m.sel_yf = yf;
+    m.sel_z = z;
+    m.sel_yi = yi;
+    m.sel_x = x;
+    m.tuned = true;
+    drop_graph(c);
+    return OSE_OK;
+}
+
+int ose_launch_shapes(ose_ctx* c, int model_id, int* strided, int* nstrided, int* xshapes, int* nxs)
+{
+    if (!c || !nstrided || !nxs) return OSE_EINVAL;
+    if (model_id < 0 || model_id >= (int)c->models.size()) return c->fail(OSE_EINVAL, "ose_launch_shapes: bad model");
+    const Model& m = *c->models[model_id];
+    // strided shapes common to the y and z extents
+    int n = 0;
+    for (int v = 0; v < m.ey->ns; ++v) {
+        const int code = m.ey->yf[v].split ? -m.ey->yf[v].T : m.ey->yf[v].T;
+        bool inz = false;
+        for (int u = 0; u < m.ez->ns; ++u) inz |= (m.ez->zs[u].split ? -m.ez->zs[u].T : m.ez->zs[u].T) == code;
+        if (inz) {
+            if (strided && n < *nstrided) strided[n] = code;
+            ++n;
+        }
+    }
+    *nstrided = n;
+    n = 0;
+    for (int v = 0; v < m.ex->nx; ++v) {
+        if (xshapes && n < *nxs) xshapes[n] = m.ex->xk[v].split ? -m.ex->xk[v].P : m.ex->xk[v].P;
+        ++n;
+    }
+    *nxs = n;
     return OSE_OK;
 }
 
@@ -1108,17 +1179,10 @@ int ose_get_wavefield(ose_ctx* c, int shot_id, float* re, float* im)
     int rc;
     if ((rc = set_device(c)) || (rc = finalize(c)) || (rc = ose_sync(c))) return rc;
     Model& m = *c->models[sh->model];
-    XArgs a = base_xargs(m, 1);
-    a.Xprev = m.X[0] + (long)sh->slot * m.ncell;
-    a.A = m.A + (long)sh->slot * m.ncell;
-    const bool prof = c->profile;
-    c->profile = false;
-    rc = launch_x(c, 0, m.ex->xk[m.sel_x], XM_INV, a, m.d.nyb * m.d.nzb, 1, 0.0);
-    c->profile = prof;
-    if (rc) return rc;
+    // the state u is kept in physical form in W[0]
     std::vector<float2> buf((size_t)m.ncell);
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(buf.data(), a.A, sizeof(float2) * buf.size(), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(buf.data(), m.W[0] + (long)sh->slot * m.ncell, sizeof(float2) * buf.size(),
+                        hipMemcpyDeviceToHost));
     for (long i = 0; i < m.ncell; ++i) {
         re[i] = buf[i].x;
         im[i] = buf[i].y;
@@ -1141,8 +1205,8 @@ int ose_set_wavefield(ose_ctx* c, int shot_id, const float* re, const float* im)
         buf[i] = make_float2(re[i], im[i]);
     }
     XArgs a = base_xargs(m, 1);
-    a.A = m.A + (long)sh->slot * m.ncell;
-    a.Xout = m.X[0] + (long)sh->slot * m.ncell;
+    a.A = m.W[0] + (long)sh->slot * m.ncell;   // u, and X_0 = FFT_x(u)
+    a.Xout = m.X + (long)sh->slot * m.ncell;
     HIPCHK(c, hipMemcpy(a.A, buf.data(), sizeof(float2) * buf.size(), hipMemcpyHostToDevice));
     const bool prof = c->profile;
     c->profile = false;

@@ -4,21 +4,23 @@
 
 namespace ose {
 
-// one compiled launch shape of a strided (y/z) pass: T x columns per workgroup
+// one compiled launch shape of a strided (y/z) pass: T x columns per
+// workgroup, split = 1 for the half-LDS (real, then imaginary) exchange
 struct StridedKern {
-    int T = 0, threads = 0;
+    int T = 0, split = 0, threads = 0;
     size_t lds = 0;
     const void* fn = nullptr;
 };
 
-// one compiled launch shape of the x pass: P lines per workgroup, a kernel per XMode
+// one compiled launch shape of the x pass: P lines per workgroup, a kernel
+// per XMode, split = 1 for the half-LDS exchange
 struct XKern {
-    int P = 0, threads = 0;
+    int P = 0, split = 0, threads = 0;
     size_t lds = 0;
     const void* fn[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 
-constexpr int kMaxVariants = 4;
+constexpr int kMaxVariants = 8;
 
 struct FftEntry {
     int N = 0, N1 = 0, N2 = 0;

@@ -9,10 +9,11 @@
 //   1. k_strided<SIGN=-1>  forward-y       X_j  -> S
 //   2. k_zsym              forward-z, * |k|/N, inverse-z   (in place on S)
 //   3. k_strided<SIGN=+1>  inverse-y       (in place on S)
-//   4. k_xpass<MODE>       inverse-x of S and of X_{j-1}, recursion, taper,
+//   4. k_xpass<MODE>       inverse-x of S, recursion on physical W, taper,
 //                          accumulator, receivers, forward-x of W_{j+1}
-// X_j is kept in x-spectral form (FFT_x W_j) so pass 4 recovers W_{j-1}
-// from X_{j-1} with an inverse x transform instead of a second stored field.
+// Passes 1-3 run in place on X_j = FFT_x W_j; pass 4 reads W_{j-1} in
+// physical form and overwrites it with W_{j+1} (two physical W buffers
+// alternate), so each order needs one inverse and one forward x transform.
 //
 // Every 1D transform of length N = N1*N2 is done in registers and LDS:
 // step A: N2 threads each run an N1-point register DFT over x[N2*n1 + t],
@@ -58,20 +59,65 @@ struct StridedArgs {
     float invN;         // 1 / (nx ny nz)
 };
 
-template <int N1, int N2, int T>
+// SPLIT = 1 exchanges the real and then the imaginary parts through an LDS
+// array of floats: half the LDS per workgroup (twice the workgroups per CU
+// for the long extents) for two more barriers.
+template <int N1, int N2, int T, int SPLIT>
 struct StridedCfg {
     static constexpr int N = N1 * N2;
     static constexpr int NT = Max<N1, N2>::value;
     static constexpr int THREADS = T * NT;
     static constexpr int LDT = T + 1;                       // padded row
-    static constexpr size_t LDS_BYTES = sizeof(float2) * (size_t)N * LDT;
+    static constexpr size_t LDS_BYTES = (SPLIT ? sizeof(float) : sizeof(float2)) * (size_t)N * LDT;
 };
 
-template <int N1, int N2, int T, int SIGN>
-__global__ __launch_bounds__((StridedCfg<N1, N2, T>::THREADS))
+// LDS exchange among the T columns' threads: writers (wr) store v[i] for
+// i < NW at row wrow(i), readers (rd) load v[i] for i < NR from row
+// rrow(i).  In split form a thread that both writes and reads overwrites
+// only the real parts of v[0..NR) before it stores its imaginary parts.
+template <int T, int SPLIT, int NW, int NR, typename FW, typename FR>
+__device__ __forceinline__ void col_exchange(float2* v, bool wr, bool rd, FW wrow, FR rrow, float2* lds, int col)
+{
+    constexpr int LDT = T + 1;
+    if (!SPLIT) {
+        if (wr) {
+#pragma unroll
+            for (int i = 0; i < NW; ++i) lds[wrow(i) * LDT + col] = v[i];
+        }
+        __syncthreads();
+        if (rd) {
+#pragma unroll
+            for (int i = 0; i < NR; ++i) v[i] = lds[rrow(i) * LDT + col];
+        }
+    } else {
+        float* f = reinterpret_cast<float*>(lds);
+        if (wr) {
+#pragma unroll
+            for (int i = 0; i < NW; ++i) f[wrow(i) * LDT + col] = v[i].x;
+        }
+        __syncthreads();
+        if (rd) {
+#pragma unroll
+            for (int i = 0; i < NR; ++i) v[i].x = f[rrow(i) * LDT + col];
+        }
+        __syncthreads();
+        if (wr) {
+#pragma unroll
+            for (int i = 0; i < NW; ++i) f[wrow(i) * LDT + col] = v[i].y;
+        }
+        __syncthreads();
+        if (rd) {
+#pragma unroll
+            for (int i = 0; i < NR; ++i) v[i].y = f[rrow(i) * LDT + col];
+        }
+    }
+}
+
+template <int N1, int N2, int T, int SIGN, int SPLIT>
+__global__ __launch_bounds__((StridedCfg<N1, N2, T, SPLIT>::THREADS))
 void k_strided(StridedArgs a)
 {
-    using C = StridedCfg<N1, N2, T>;
+    using C = StridedCfg<N1, N2, T, SPLIT>;
     extern __shared__ float2 lds[];
     const int tid = threadIdx.x;
     const int col = tid % T;
@@ -90,13 +136,10 @@ void k_strided(StridedArgs a)
             const float2 w = a.tw[t * k1];
             v[k1] = SIGN < 0 ? cmul(v[k1], w) : cmulc(v[k1], w);
         }
-#pragma unroll
-        for (int k1 = 0; k1 < N1; ++k1) lds[(t * N1 + k1) * C::LDT + col] = v[k1];
     }
-    __syncthreads();
+    col_exchange<T, SPLIT, N1, N2>(
+        v, t < N2, t < N1, [&](int k1) { return t * N1 + k1; }, [&](int n2) { return n2 * N1 + t; }, lds, col);
     if (t < N1) {
-#pragma unroll
-        for (int n2 = 0; n2 < N2; ++n2) v[n2] = lds[(n2 * N1 + t) * C::LDT + col];
         rdft<N2, SIGN>(v);
         if (ok) {
 #pragma unroll
@@ -107,12 +150,14 @@ void k_strided(StridedArgs a)
 
 // z pass: forward-z, multiply by |k|/N (folded wavenumbers, scale_kxkz_3D
 // ose_3d_kernel.cu:101-126 and the 1/N of scal_vr_3d :355), inverse-z.
-// In place; blockIdx.y is the (spectral) y index.
-template <int N1, int N2, int T>
-__global__ __launch_bounds__((StridedCfg<N1, N2, T>::THREADS))
+// In place; blockIdx.y is the (spectral) y index.  Each LDS location of the
+// second exchange is written by the thread that read it in the first, so
+// the exchanges need no barrier between them.
+template <int N1, int N2, int T, int SPLIT>
+__global__ __launch_bounds__((StridedCfg<N1, N2, T, SPLIT>::THREADS))
 void k_zsym(StridedArgs a)
 {
-    using C = StridedCfg<N1, N2, T>;
+    using C = StridedCfg<N1, N2, T, SPLIT>;
     extern __shared__ float2 lds[];
     const int tid = threadIdx.x;
     const int col = tid % T;
@@ -128,13 +173,10 @@ void k_zsym(StridedArgs a)
         rdft<N1, -1>(v);
 #pragma unroll
         for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], a.tw[t * k1]);
-#pragma unroll
-        for (int k1 = 0; k1 < N1; ++k1) lds[(t * N1 + k1) * C::LDT + col] = v[k1];
     }
-    __syncthreads();
+    col_exchange<T, SPLIT, N1, N2>(
+        v, t < N2, t < N1, [&](int k1) { return t * N1 + k1; }, [&](int n2) { return n2 * N1 + t; }, lds, col);
     if (t < N1) {
-#pragma unroll
-        for (int n2 = 0; n2 < N2; ++n2) v[n2] = lds[(n2 * N1 + t) * C::LDT + col];
         rdft<N2, -1>(v);
         const float kxy = (ok ? a.kx2[x] : 0.f) + a.ky2[blockIdx.y];
 #pragma unroll
@@ -147,13 +189,10 @@ void k_zsym(StridedArgs a)
         rdft<N2, 1>(v);
 #pragma unroll
         for (int n2 = 1; n2 < N2; ++n2) v[n2] = cmulc(v[n2], a.tw[t * n2]);
-#pragma unroll
-        for (int n2 = 0; n2 < N2; ++n2) lds[(n2 * N1 + t) * C::LDT + col] = v[n2];
     }
-    __syncthreads();
+    col_exchange<T, SPLIT, N2, N1>(
+        v, t < N1, t < N2, [&](int n2) { return n2 * N1 + t; }, [&](int m1) { return t * N1 + m1; }, lds, col);
     if (t < N2) {
-#pragma unroll
-        for (int m1 = 0; m1 < N1; ++m1) v[m1] = lds[(t * N1 + m1) * C::LDT + col];
         rdft<N1, 1>(v);
         if (ok) {
 #pragma unroll
@@ -166,19 +205,25 @@ void k_zsym(StridedArgs a)
 // x pass: a workgroup owns P consecutive x lines of one shot.
 // ---------------------------------------------------------------------
 enum XMode : int {
-    XM_ORDER0 = 0,   // W1 = -i L W0;  A = c0 W0 + c1 W1;  out X1
-    XM_MID = 1,      // W' = W_{j-1} - 2i L W_j;  A = g (A + c W');  out X'
-    XM_LAST = 2,     // as MID, u = g (A + c W') is the new state: receivers, out X0
-    XM_FWD = 3,      // out = FFT_x(A)              (state upload)
-    XM_INV = 4,      // A = IFFT_x(Xprev) / nx      (state download)
-    XM_LASTA = 5,    // as LAST, but the new state u goes to A (physical) for the boundary network
+    XM_ORDER0 = 0,   // W1 = -i L W0;  A = c0 W0 + c1 W1;  out W1, X1
+    XM_MID = 1,      // W' = W_{j-1} - 2i L W_j;  A = g (A + c W');  out W', X'
+    XM_LAST = 2,     // as MID, u = g (A + c W') is the new state: receivers, out u, X0
+    XM_FWD = 3,      // Xout = FFT_x(A)              (state upload)
+    XM_INV = 4,      // A = IFFT_x(Xprev) / nx       (transform utilities)
+    XM_LASTA = 5,    // as LAST without the forward transform (the boundary network acts on u first)
 };
 
+// The recursion runs on physical W: pass 4 of order j reads W_{j-1} and
+// writes W_{j+1} over it (Win == Wout except where noted), so only the
+// spectrum S needs an inverse x transform; the new W is forward transformed
+// into X for the next order's passes 1-3, which run in place on X.
 struct XArgs {
     const float2* S;        // spectrum after passes 1-3 (natural order along x)
-    const float2* Xprev;    // x-spectral W_{j-1} (W_0 in ORDER0)
+    const float2* Win;      // physical W_{j-1} (W_0 in ORDER0)
+    float2* Wout;           // physical W_{j+1} (W_1 in ORDER0, the new state u in LAST/LASTA)
     float2* Xout;           // x-spectral W_{j+1} (X_0 of the next step in LAST)
-    float2* A;              // accumulator / physical state
+    float2* A;              // accumulator
+    const float2* Xprev;    // XM_INV input
     const float* vr;        // v / R, [z][y][x], shared by the shots of a model
     const float* gx;
     const float* gy;
@@ -198,21 +243,64 @@ struct XArgs {
     int nt_cap;
 };
 
-template <int N1, int N2, int P>
+template <int N1, int N2, int P, int SPLIT>
 struct XCfg {
     static constexpr int N = N1 * N2;
     static constexpr int NT = Max<N1, N2>::value;
     static constexpr int THREADS = P * NT;
     static constexpr int PN = N1 | 1;                 // odd LDS pitch
-    static constexpr int LINE = N2 * PN;
-    static constexpr size_t LDS_BYTES = sizeof(float2) * (size_t)P * LINE;
+    static constexpr int LINE = N2 * PN;              // elements (float2, or float when split)
+    static constexpr size_t LDS_BYTES = (SPLIT ? sizeof(float) : sizeof(float2)) * (size_t)P * LINE;
 };
+
+// LDS exchange within one x line: writers store src[i] (i < NW) at wpos(i),
+// readers load dst[i] (i < NR) from rpos(i); src and dst are different
+// register arrays.  SPLIT = 1 moves real and imaginary parts in two rounds
+// through a float array of half the size.
+template <int SPLIT, int NW, int NR, typename FW, typename FR>
+__device__ __forceinline__ void line_exchange(const float2* src, bool wr, float2* dst, bool rd, FW wpos, FR rpos,
+                                              void* Lv)
+{
+    if (!SPLIT) {
+        float2* L = reinterpret_cast<float2*>(Lv);
+        if (wr) {
+#pragma unroll
+            for (int i = 0; i < NW; ++i) L[wpos(i)] = src[i];
+        }
+        __syncthreads();
+        if (rd) {
+#pragma unroll
+            for (int i = 0; i < NR; ++i) dst[i] = L[rpos(i)];
+        }
+    } else {
+        float* f = reinterpret_cast<float*>(Lv);
+        if (wr) {
+#pragma unroll
+            for (int i = 0; i < NW; ++i) f[wpos(i)] = src[i].x;
+        }
+        __syncthreads();
+        if (rd) {
+#pragma unroll
+            for (int i = 0; i < NR; ++i) dst[i].x = f[rpos(i)];
+        }
+        __syncthreads();
+        if (wr) {
+#pragma unroll
+            for (int i = 0; i < NW; ++i) f[wpos(i)] = src[i].y;
+        }
+        __syncthreads();
+        if (rd) {
+#pragma unroll
+            for (int i = 0; i < NR; ++i) dst[i].y = f[rpos(i)];
+        }
+    }
+}
 
 // inverse x transform of one line: thread t < N1 loads X[t + N1 m2]; on
 // return thread t < N2 holds x[N2 n1 + t] (unnormalised) in res[n1].
-template <int N1, int N2, int PN>
+template <int N1, int N2, int PN, int SPLIT>
 __device__ __forceinline__ void x_inverse(const float2* __restrict__ src, bool okl, int t,
-                                          float2* __restrict__ L, const float2* __restrict__ tw,
+                                          void* L, const float2* __restrict__ tw,
                                           float2* __restrict__ w, float2* __restrict__ res)
 {
     if (t < N1) {
@@ -221,36 +309,28 @@ __device__ __forceinline__ void x_inverse(const float2* __restrict__ src, bool o
         rdft<N2, 1>(w);
 #pragma unroll
         for (int n2 = 1; n2 < N2; ++n2) w[n2] = cmulc(w[n2], tw[t * n2]);
-#pragma unroll
-        for (int n2 = 0; n2 < N2; ++n2) L[n2 * PN + t] = w[n2];
     }
-    __syncthreads();
-    if (t < N2) {
-#pragma unroll
-        for (int m1 = 0; m1 < N1; ++m1) res[m1] = L[t * PN + m1];
-        rdft<N1, 1>(res);
-    }
+    line_exchange<SPLIT, N2, N1>(
+        w, t < N1, res, t < N2, [&](int n2) { return n2 * PN + t; }, [&](int m1) { return t * PN + m1; }, L);
+    if (t < N2) rdft<N1, 1>(res);
     __syncthreads();
 }
 
 // forward x transform: thread t < N2 holds x[N2 n1 + t] in u[n1]; writes
 // X[k] to dst (natural order).
-template <int N1, int N2, int PN>
+template <int N1, int N2, int PN, int SPLIT>
 __device__ __forceinline__ void x_forward(float2* __restrict__ u, bool okl, int t,
-                                          float2* __restrict__ L, const float2* __restrict__ tw,
+                                          void* L, const float2* __restrict__ tw,
                                           float2* __restrict__ w, float2* dst)
 {
     if (t < N2) {
         rdft<N1, -1>(u);
 #pragma unroll
         for (int k1 = 1; k1 < N1; ++k1) u[k1] = cmul(u[k1], tw[t * k1]);
-#pragma unroll
-        for (int k1 = 0; k1 < N1; ++k1) L[t * PN + k1] = u[k1];
     }
-    __syncthreads();
+    line_exchange<SPLIT, N1, N2>(
+        u, t < N2, w, t < N1, [&](int k1) { return t * PN + k1; }, [&](int n2) { return n2 * PN + t; }, L);
     if (t < N1) {
-#pragma unroll
-        for (int n2 = 0; n2 < N2; ++n2) w[n2] = L[n2 * PN + t];
         rdft<N2, -1>(w);
         if (okl) {
 #pragma unroll
@@ -259,11 +339,11 @@ __device__ __forceinline__ void x_forward(float2* __restrict__ u, bool okl, int 
     }
 }
 
-template <int N1, int N2, int P, int MODE>
-__global__ __launch_bounds__((XCfg<N1, N2, P>::THREADS))
+template <int N1, int N2, int P, int MODE, int SPLIT>
+__global__ __launch_bounds__((XCfg<N1, N2, P, SPLIT>::THREADS))
 void k_xpass(XArgs a)
 {
-    using C = XCfg<N1, N2, P>;
+    using C = XCfg<N1, N2, P, SPLIT>;
     constexpr int N = C::N;
     extern __shared__ float2 lds[];
     const int l = threadIdx.x / C::NT;
@@ -274,7 +354,8 @@ void k_xpass(XArgs a)
     const int z = okl ? (int)(line / a.ny) : 0;
     const long lbase = (long)blockIdx.y * a.shot_stride + line * N;
     const long vbase = line * N;
-    float2* L = lds + l * C::LINE;
+    // this line's exchange area: LINE float2 elements, or LINE floats when split
+    void* L = SPLIT ? (void*)(reinterpret_cast<float*>(lds) + l * C::LINE) : (void*)(lds + l * C::LINE);
     float2 u[N1];
     float2 w[C::NT];
 
@@ -283,11 +364,11 @@ void k_xpass(XArgs a)
 #pragma unroll
             for (int n1 = 0; n1 < N1; ++n1) u[n1] = a.A[lbase + N2 * n1 + t];
         }
-        x_forward<N1, N2, C::PN>(u, okl, t, L, a.tw, w, a.Xout + lbase);
+        x_forward<N1, N2, C::PN, SPLIT>(u, okl, t, L, a.tw, w, a.Xout + lbase);
         return;
     }
     if (MODE == XM_INV) {
-        x_inverse<N1, N2, C::PN>(a.Xprev + lbase, okl, t, L, a.tw, w, u);
+        x_inverse<N1, N2, C::PN, SPLIT>(a.Xprev + lbase, okl, t, L, a.tw, w, u);
         if (t < N2 && okl) {
 #pragma unroll
             for (int n1 = 0; n1 < N1; ++n1)
@@ -296,10 +377,8 @@ void k_xpass(XArgs a)
         return;
     }
 
-    // L W_j in physical x (unnormalised inverse; the symbol carried 1/N)
-    x_inverse<N1, N2, C::PN>(a.S + lbase, okl, t, L, a.tw, w, u);
-    float2 p[N1];
-    x_inverse<N1, N2, C::PN>(a.Xprev + lbase, okl, t, L, a.tw, w, p);
+    // L W_j / (v/R) in physical x (unnormalised inverse; the symbol carried 1/N)
+    x_inverse<N1, N2, C::PN, SPLIT>(a.S + lbase, okl, t, L, a.tw, w, u);
 
     if (t < N2 && okl) {
         const float gyz = a.gy[y] * a.gz[z];
@@ -309,47 +388,59 @@ void k_xpass(XArgs a)
             itv = *a.it;
             rec = (z == a.rec_z) && (y >= a.rec_y0) && (y < a.rec_y0 + a.rec_ny) && (itv < a.nt_cap);
         }
+        // Win and Wout may be the same buffer (each element is read and then
+        // written by this thread only): every Win load precedes the Wout
+        // stores, which are issued after the loop; A never aliases Win.
+        const float2* win = a.Win + lbase;
+        float2* __restrict__ acc = a.A + lbase;
 #pragma unroll
         for (int n1 = 0; n1 < N1; ++n1) {
             const int x = N2 * n1 + t;
             const float vr = a.vr[vbase + x];
+            const float2 wp = win[x];
             const float2 lw = make_float2(vr * u[n1].x, vr * u[n1].y);
-            const float2 wp = make_float2(p[n1].x * a.inv_nx, p[n1].y * a.inv_nx);
             float2 wn;
             if (MODE == XM_ORDER0) {
                 // W1 = -i L W0
                 wn = make_float2(lw.y, -lw.x);
-                a.A[lbase + x] = make_float2(a.c0 * wp.x + a.c1 * wn.x, a.c0 * wp.y + a.c1 * wn.y);
+                acc[x] = make_float2(a.c0 * wp.x + a.c1 * wn.x, a.c0 * wp.y + a.c1 * wn.y);
             } else {
                 // W_{j+1} = W_{j-1} - 2i L W_j
                 wn = make_float2(wp.x + 2.f * lw.y, wp.y - 2.f * lw.x);
                 const float g = gyz * a.gx[x];
-                const float2 acc = a.A[lbase + x];
-                const float2 an = make_float2(g * (acc.x + a.c1 * wn.x), g * (acc.y + a.c1 * wn.y));
-                if (MODE == XM_MID || MODE == XM_LASTA) {
-                    a.A[lbase + x] = an;
-                }
-                if (MODE == XM_LAST || MODE == XM_LASTA) {
+                const float2 ao = acc[x];
+                const float2 an = make_float2(g * (ao.x + a.c1 * wn.x), g * (ao.y + a.c1 * wn.y));
+                if (MODE == XM_MID) acc[x] = an;
+                if (MODE == XM_LAST || MODE == XM_LASTA)
                     wn = an;   // the new state u; its x transform is X_0 of the next step
-                    if (rec && x >= a.rec_x0 && x < a.rec_x0 + a.rec_nx)
-                        a.traces[(long)blockIdx.y * a.trace_shot_stride +
-                                 ((long)itv * a.rec_ny + (y - a.rec_y0)) * a.rec_nx + (x - a.rec_x0)] = an.y;
-                }
             }
             u[n1] = wn;
         }
+        float2* wout = a.Wout + lbase;
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) wout[N2 * n1 + t] = u[n1];
+        if ((MODE == XM_LAST || MODE == XM_LASTA) && rec) {
+            // receivers: Im(u) on the plane z = rec_z
+            float* tr = a.traces + (long)blockIdx.y * a.trace_shot_stride + ((long)itv * a.rec_ny + (y - a.rec_y0)) * a.rec_nx;
+#pragma unroll
+            for (int n1 = 0; n1 < N1; ++n1) {
+                const int x = N2 * n1 + t;
+                if (x >= a.rec_x0 && x < a.rec_x0 + a.rec_nx) tr[x - a.rec_x0] = u[n1].y;
+            }
+        }
     }
-    if (MODE != XM_LASTA) x_forward<N1, N2, C::PN>(u, okl, t, L, a.tw, w, a.Xout + lbase);
+    if (MODE != XM_LASTA) x_forward<N1, N2, C::PN, SPLIT>(u, okl, t, L, a.tw, w, a.Xout + lbase);
 }
 
 // ---------------------------------------------------------------------
 // Source injection in x-spectral form and the step counter.
 // u[src] += s[it] + i h[it] before each step (ose_3d.cu:252-253,
 // add_wavelet_Xt ose_3d_kernel.cu:291-297) becomes, on X_0 = FFT_x(u),
-// X_0[line][k] += (s + i h) exp(-2 pi i k sx / nx).
+// X_0[line][k] += (s + i h) exp(-2 pi i k sx / nx), and W_0[src] += s + i h.
 // ---------------------------------------------------------------------
 struct InjectArgs {
     float2* X0;
+    float2* W0;             // the same state in physical form
     long shot_stride;
     int nx;
     const long* src_line;   // per shot: (sz*ny + sy) * nx

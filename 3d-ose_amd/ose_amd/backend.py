@@ -98,9 +98,22 @@ class Context:
         self._check(self._L.ose_set_autotune(self._h, int(on)))
 
     def tuning(self, model):
+        """Launch shapes in use: T per strided pass, P of the x pass
+        (negative: half-LDS split exchange)."""
         out = (c_int * 4)()
         self._check(self._L.ose_get_tuning(self._h, model, out))
         return dict(zip(("T_fwd_y", "T_z", "T_inv_y", "P_x"), list(out)))
+
+    def set_tuning(self, model, t_fwd_y, t_z, t_inv_y, p_x):
+        arr = (c_int * 4)(t_fwd_y, t_z, t_inv_y, p_x)
+        self._check(self._L.ose_set_tuning(self._h, model, arr))
+
+    def launch_shapes(self, model):
+        """(strided codes usable on y and z, x-pass codes) compiled for a model."""
+        ns, nx = c_int(16), c_int(16)
+        st, xs = (c_int * 16)(), (c_int * 16)()
+        self._check(self._L.ose_launch_shapes(self._h, model, st, ctypes.byref(ns), xs, ctypes.byref(nx)))
+        return list(st)[:ns.value], list(xs)[:nx.value]
 
     def set_boundary_net(self, model, params, guard=2.0, replace_taper=True):
         """Enable the learned thin pad on a model (None removes it)."""

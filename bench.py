@@ -35,12 +35,12 @@ CONFIGS = {
     "ref": ("preset=ref", "reference as shipped: 101^3 two-layer, 25-cell pads"),
 }
 PASS_BYTES_DOC = {
-    "fwd_y": "16 B/cell (read X_j, write S)",
-    "z_symbol": "16 B/cell (read S, write S)",
-    "inv_y": "16 B/cell (read S, write S)",
-    "x_epilogue": "44 B/cell (read S, X_{j-1}, v/R, A; write X_{j+1}, A)",
-    "x_first": "36 B/cell (read S, X_0, v/R; write X_1, A)",
-    "x_last": "36 B/cell (read S, X_{K-2}, v/R, A; write X_0)",
+    "fwd_y": "16 B/cell (read X_j, write it in place)",
+    "z_symbol": "16 B/cell (read, write in place)",
+    "inv_y": "16 B/cell (read, write in place: S)",
+    "x_epilogue": "52 B/cell (read S, W_{j-1}, v/R, A; write X_{j+1}, W_{j+1}, A)",
+    "x_first": "44 B/cell (read S, W_0, v/R; write X_1, W_1, A)",
+    "x_last": "44 B/cell (read S, W_{K-2}, v/R, A; write X_0, u)",
     "inject": "16 B per x cell of the source line",
 }
 HBM_PEAK_GBS = 8000.0

@@ -127,8 +127,17 @@ int ose_reset_stats(ose_ctx* ctx);
  * strided passes, lines per workgroup for the x pass) and the fastest kept.
  * No counterpart in the reference (cuFFT chose its own plan, ose_3d.cu:226). */
 int ose_set_autotune(ose_ctx* ctx, int on);
-/* shape[4] = {T forward-y, T z-symbol, T inverse-y, P x-pass} of a model.  */
+/* shape[4] = {T forward-y, T z-symbol, T inverse-y, P x-pass} of a model;
+ * a negative value marks the half-LDS (split real/imaginary) exchange.    */
 int ose_get_tuning(ose_ctx* ctx, int model_id, int* shape);
+/* Pin a model's launch shapes (same encoding as ose_get_tuning; the
+ * strided entries must be compiled for the y and z extents).  Skips the
+ * autotuner for that model.  No counterpart in the reference.             */
+int ose_set_tuning(ose_ctx* ctx, int model_id, const int* shape);
+/* The compiled launch shapes of a model: strided codes usable on both the y
+ * and z passes, and x-pass codes.  On entry *nstrided, *nxs are the array
+ * capacities; on return, the counts.  No counterpart in the reference.   */
+int ose_launch_shapes(ose_ctx* ctx, int model_id, int* strided, int* nstrided, int* xshapes, int* nxs);
 
 /* Bandwidth yardstick: a plain 16-byte-per-lane device copy of `bytes`,
  * repeated reps times; gbs = (read + write bytes) / time.  No counterpart in

@@ -108,5 +108,6 @@ def test_run_matches_oracle(ose, graph):
         tr = c.traces(sid)
         re, im = c.wavefield(sid)
         assert c.step == nt
+    print(f"traces {rel(tr, tr_ref):.3g} field {rel(re + 1j * im, P + 1j * Q):.3g}", flush=True)
     assert rel(tr, tr_ref) < 1e-4
     assert rel(re + 1j * im, P + 1j * Q) < 1e-4
