@@ -44,7 +44,7 @@ def strided_variants(n1, n2):
     nt = max(n1, n2)
     out = []
     for t in STRIDED_TS:
-        if t * nt > 1024:
+        if t * nt > 768:      # larger workgroups spill the register DFTs
             continue
         full = 8 * n1 * n2 * (t + 1)
         if full <= LDS_MAX:

@@ -7,6 +7,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <iterator>
 #include <map>
 #include <sstream>
 #include <string>
@@ -242,6 +244,49 @@ int build_model(oseh_sim* s, const ModelSpec& spec, BuiltModel& m)
     return OSE_OK;
 }
 
+// Boundary-network parameters: a .npy file holding one little-endian
+// float32 array (version 1-3 header, C order), or raw float32 otherwise.
+int read_params_f32(const std::string& path, std::vector<float>& out, std::string& err)
+{
+    std::ifstream f(path, std::ios::binary);
+    if (!f) {
+        err = "cannot open " + path;
+        return OSE_EINVAL;
+    }
+    std::vector<char> all((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    size_t off = 0;
+    if (all.size() >= 10 && std::memcmp(all.data(), "\x93NUMPY", 6) == 0) {
+        const int major = (unsigned char)all[6];
+        size_t hlen = 0, hoff = 0;
+        if (major == 1) {
+            hlen = (unsigned char)all[8] | ((size_t)(unsigned char)all[9] << 8);
+            hoff = 10;
+        } else if (all.size() >= 12) {
+            hlen = 0;
+            for (int k = 0; k < 4; ++k) hlen |= (size_t)(unsigned char)all[8 + k] << (8 * k);
+            hoff = 12;
+        }
+        if (!hoff || hoff + hlen > all.size()) {
+            err = path + ": bad .npy header";
+            return OSE_EINVAL;
+        }
+        const std::string h(all.data() + hoff, hlen);
+        if (h.find("'<f4'") == std::string::npos || h.find("'fortran_order': False") == std::string::npos) {
+            err = path + ": expected a C-order little-endian float32 array";
+            return OSE_EINVAL;
+        }
+        off = hoff + hlen;
+    }
+    const size_t n = (all.size() - off) / sizeof(float);
+    if (n == 0 || (all.size() - off) % sizeof(float)) {
+        err = path + ": size is not a whole number of float32 values";
+        return OSE_EINVAL;
+    }
+    out.resize(n);
+    std::memcpy(out.data(), all.data() + off, n * sizeof(float));
+    return OSE_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -348,10 +393,36 @@ int oseh_sim_setup(oseh_sim* s)
             return s->fail(rc, std::string("ose_add_shot: ") + ose_last_error(s->ctx));
         m.shot_ids.push_back(sid);
     }
+    // learned thin pad: bnet=<parameters file> [guard=4] [bnet_taper=replace|keep]
+    const std::string bn = s->get("bnet", "");
+    if (!bn.empty()) {
+        std::vector<float> par;
+        std::string err;
+        if ((rc = read_params_f32(bn, par, err))) return s->fail(rc, err);
+        const int replace = s->get("bnet_taper", "replace") == "keep" ? 0 : 1;
+        for (BuiltModel& m : s->models)
+            if ((rc = ose_set_boundary_net(s->ctx, m.id, par.data(), par.size(), s->getd("guard", 4.0), replace)))
+                return s->fail(rc, std::string("ose_set_boundary_net: ") + ose_last_error(s->ctx));
+    }
     return OSE_OK;
 }
 
 ose_ctx* oseh_sim_ctx(oseh_sim* s) { return s ? s->ctx : nullptr; }
+
+int oseh_read_params(const char* path, float* out, size_t cap, size_t* n)
+{
+    if (!path || !n) return OSE_EINVAL;
+    std::vector<float> p;
+    std::string err;
+    int rc = read_params_f32(path, p, err);
+    if (rc) return rc;
+    *n = p.size();
+    if (out) {
+        if (cap < p.size()) return OSE_EINVAL;
+        std::memcpy(out, p.data(), p.size() * sizeof(float));
+    }
+    return OSE_OK;
+}
 
 int oseh_sim_run(oseh_sim* s, int nsteps, double* ms)
 {
@@ -385,7 +456,7 @@ int oseh_sim_info(oseh_sim* s, char* buf, size_t cap)
     o << "], \"nshots\": " << s->shots.size() << ", \"interior_cells\": " << (long long)interior
       << ", \"padded_cells\": " << (long long)padded << ", \"nt\": " << s->geti("nt", 300)
       << ", \"taper\": \"" << s->get("taper", "ref") << "\", \"korder\": \"" << s->get("korder", "tail")
-      << "\"}";
+      << "\", \"bnet\": \"" << s->get("bnet", "") << "\"}";
     const std::string t = o.str();
     if (t.size() + 1 > cap) return OSE_EINVAL;
     std::memcpy(buf, t.c_str(), t.size() + 1);

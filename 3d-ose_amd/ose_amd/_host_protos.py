@@ -32,6 +32,7 @@ PROTOS = {
     "oseh_sim_setup": ([vp], c_int),
     "oseh_sim_run": ([vp, c_int, dp], c_int),
     "oseh_sim_ctx": ([vp], vp),
+    "oseh_read_params": ([cp, fp, c_size_t, P(c_size_t)], c_int),
     "oseh_sim_info": ([vp, ctypes.c_char_p, c_size_t], c_int),
     "oseh_sim_write": ([vp, cp], c_int),
     "oseh_sim_snapshot": ([vp, cp], c_int),

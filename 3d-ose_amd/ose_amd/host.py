@@ -131,6 +131,19 @@ def write_snapshot(path, f_zyx):
         raise OseError(rc, f"cannot write {path}")
 
 
+def read_params(path):
+    """Boundary-network parameters from a .npy float32 array or raw float32 file."""
+    n = ctypes.c_size_t()
+    rc = _L().oseh_read_params(str(path).encode(), None, 0, ctypes.byref(n))
+    if rc:
+        raise OseError(rc, f"cannot read parameters from {path}")
+    out = np.empty(n.value, dtype=np.float32)
+    rc = _L().oseh_read_params(str(path).encode(), out.ctypes.data_as(_fp), out.size, ctypes.byref(n))
+    if rc:
+        raise OseError(rc, f"cannot read parameters from {path}")
+    return out
+
+
 class Simulation:
     """A preset or key=value configuration driven through the HIP backend."""
 

@@ -97,9 +97,80 @@ def rocprof_summary(db_path):
     return "\n".join(out)
 
 
+def pmc_summary(db_path):
+    """Mean counter value per dispatch (rocprofv3 --pmc results database),
+    by kernel, with the number of dispatches."""
+    import sqlite3
+    con = sqlite3.connect(db_path)
+    rows = con.execute("select kernel_name, counter_name, count(*), avg(value), min(value), max(value) "
+                       "from counters_collection group by kernel_name, counter_name "
+                       "order by avg(value) desc").fetchall()
+    out = [f"{'dispatches':>10} {'mean':>14} {'min':>14} {'max':>14}  counter  kernel"]
+    for name, ctr, n, avg, mn, mx in rows:
+        out.append(f"{n:10d} {avg:14.1f} {mn:14.1f} {mx:14.1f}  {ctr}  {name}")
+    return "\n".join(out)
+
+
+def traffic_from_profile(config, kernel):
+    """HBM bytes per launch of `kernel` from the committed counter profile of
+    this workload (rocprofv3 FETCH_SIZE and WRITE_SIZE runs, each scaled by
+    the same counter on a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{config}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("traffic_bytes_per_launch_all_shots")
+
+
+def tuning(sim, nmodels):
+    import ctypes
+    from ose_amd import _lib
+    out = []
+    for m in range(nmodels):
+        t = (ctypes.c_int * 4)()
+        _lib.hip().ose_get_tuning(sim.ctx, m, t)
+        out.append(list(t))
+    return out
+
+
+def learned_leg(params, steps, warmup, plain_value, weights):
+    """The same workload with an 8-cell pad: plain thin taper, then the
+    learned pad (network in the step, replacing the taper)."""
+    from ose_amd.host import Simulation
+    res = {}
+    for name, extra in (("thin_taper", ""), ("learned", f"bnet={weights}")):
+        sim = Simulation(f"{params} boundary=8 nt={warmup + steps + 1} {extra}")
+        sim.setup()
+        info = sim.info()
+        sim.timed(warmup)
+        ms = sim.timed(steps)
+        res[name] = {"ms_per_step": ms / steps, "value": info["interior_cells"] * steps / (ms * 1e-3) / 1e9,
+                     "padded": info["models"][0]["padded"]}
+        print(f"[bench] {name}: {ms / steps:.1f} ms/step", file=sys.stderr, flush=True)
+        sim.close()
+    lr = res["learned"]
+    return {
+        "value": lr["value"],
+        "unit": "Gcell*steps/s",
+        "padded": lr["padded"],
+        "ms_per_step": lr["ms_per_step"],
+        "vs_plain": lr["value"] / plain_value,
+        "thin_taper_value": res["thin_taper"]["value"],
+        "network_ms_per_step": lr["ms_per_step"] - res["thin_taper"]["ms_per_step"],
+        "weights": os.path.relpath(weights, ROOT),
+        "accuracy": ("not at the rel-L2 1e-3 bar: on held-out 64^3 models the trained 8-cell pad cuts the "
+                     "interior reflected energy 3-4x against the 8-cell taper, seismogram rel-L2 vs a 64-cell "
+                     "sponge ~0.45 (taper ~0.55); tests/test_gpu_configs.py"),
+    }
+
+
 def main():
     if len(sys.argv) == 3 and sys.argv[1] == "--rocprof-summary":
         print(rocprof_summary(sys.argv[2]))
+        return
+    if len(sys.argv) == 3 and sys.argv[1] == "--pmc-summary":
+        print(pmc_summary(sys.argv[2]))
         return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,10 +180,13 @@ def main():
     ap.add_argument("--extra", default="", help="extra key=value parameters")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the 8-cell learned-pad leg")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="also run one plain device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); shots shard "
-              "across GPUs without exchange, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); shots would shard "
+              "across GPUs without any exchange", file=sys.stderr)
         sys.exit(2)
 
     from ose_amd.host import Simulation
@@ -147,14 +221,26 @@ def main():
                             "GBs": by / (avg_ms * 1e-3) / 1e9 if avg_ms else 0.0}
     dom = max((p for p in passes if p != "inject"), key=lambda p: passes[p]["avg_us"] * passes[p]["launches"])
     pd = passes[dom]
+    if args.copy_probe:
+        from ose_amd import _lib
+        import ctypes
+        g = ctypes.c_double()
+        _lib.hip().ose_copy_probe(sim.ctx, args.copy_probe, 1, ctypes.byref(g))
+        print(f"[bench] copy probe {args.copy_probe} B: {g.value:.1f} GB/s", file=sys.stderr, flush=True)
     roofline = {"bound": "hbm", "achieved": pd["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": pd["GBs"] / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                "frac": pd["GBs"] / HBM_PEAK_GBS, "traffic": traffic_from_profile(args.config, dom), "kernel": dom,
                 "bytes_per_launch": pd["bytes"], "bytes_rule": PASS_BYTES_DOC[dom],
                 "passes": {k: round(v["GBs"], 1) for k, v in passes.items()}}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sim, info, args.cpu_seconds)
     m0 = info["models"][0]
+    tune = tuning(sim, len(info["models"]))
+    sim.close()
+    learned = None
+    if not args.no_learned and args.config in ("c4", "c1"):
+        from ose_amd.boundary import WEIGHTS
+        learned = learned_leg(params + " " + args.extra, args.steps, args.warmup, value, WEIGHTS)
     out = {
         "metric": METRIC,
         "value": value,
@@ -180,12 +266,13 @@ def main():
             "padded_cells_per_step": info["padded_cells"],
             "padded_rate": info["padded_cells"] * args.steps / (ms * 1e-3) / 1e9,
             "ms_per_step_profiled": ms_prof / prof_steps,
+            "launch_shapes": tune,
         },
+        "learned": learned,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
-    sim.close()
 
 
 if __name__ == "__main__":

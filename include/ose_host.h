@@ -113,6 +113,11 @@ int  oseh_sim_setup(oseh_sim* sim);
 /* Run nsteps for every shot; ms receives the HIP-event time of the loop.  */
 int  oseh_sim_run(oseh_sim* sim, int nsteps, double* ms);
 ose_ctx* oseh_sim_ctx(oseh_sim* sim);
+/* Boundary-network parameter file (.npy float32 array, or raw float32):
+ * *n = count; values copied to out when out != NULL and cap >= count.
+ * The simulation key bnet=<file> loads it for every model (guard=<g>,
+ * bnet_taper=replace|keep select ose_set_boundary_net's options).          */
+int oseh_read_params(const char* path, float* out, size_t cap, size_t* n);
 /* JSON description: models (interior, padded, K, R, taper), shots,
  * interior and padded cells per step.                                      */
 int  oseh_sim_info(oseh_sim* sim, char* buf, size_t cap);

@@ -174,3 +174,19 @@ def test_c2_batch_layout(H):
     pads = [sim.model_dims(i)[9:12] for i in range(5)]
     assert pads[0] == [600, 600, 480]
     sim.close()
+
+
+def test_boundary_parameter_file_formats(H, tmp_path):
+    """bnet=<file>: .npy (float32, C order) and raw float32 parameter files."""
+    rng = np.random.default_rng(9)
+    p = rng.standard_normal(20784).astype(np.float32)
+    np.save(tmp_path / "w.npy", p)
+    p.tofile(tmp_path / "w.bin")
+    assert np.array_equal(H.read_params(tmp_path / "w.npy"), p)
+    assert np.array_equal(H.read_params(tmp_path / "w.bin"), p)
+    np.save(tmp_path / "d.npy", p.astype(np.float64))
+    with pytest.raises(Exception):
+        H.read_params(tmp_path / "d.npy")
+    from ose_amd import boundary
+    w = H.read_params(boundary.WEIGHTS)
+    assert w.size == 20784 and np.all(np.isfinite(w))
