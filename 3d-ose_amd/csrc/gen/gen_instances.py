@@ -55,17 +55,20 @@ def strided_variants(n1, n2):
 
 
 def x_variants(n1, n2):
-    """(P, split) launch shapes: P lines per workgroup; the split (half-LDS)
-    exchange where a line's whole-complex exchange exceeds 2 KiB."""
+    """(P, split, twb) launch shapes: P lines per workgroup; the split
+    (half-LDS) exchange where a line's whole-complex exchange exceeds 2 KiB;
+    twb = 1 (scheduling barrier before the twiddle multiplies) for extents
+    of 600 and more, where register pressure decides the occupancy."""
     nt = max(n1, n2)
     pn = n1 | 1
     out = []
-    for p in sorted({max(1, 128 // nt), max(1, 256 // nt)}):
-        full = 8 * p * n2 * pn
-        if p * nt <= 1024 and full <= LDS_MAX:
-            out.append((p, 0))
-            if 8 * n2 * pn > 2048:
-                out.append((p, 1))
+    for twb in ((0, 1) if n1 * n2 >= 600 else (0,)):
+        for p in sorted({max(1, 128 // nt), max(1, 256 // nt)}):
+            full = 8 * p * n2 * pn
+            if p * nt <= 1024 and full <= LDS_MAX:
+                out.append((p, 0, twb))
+                if 8 * n2 * pn > 2048:
+                    out.append((p, 1, twb))
     return out
 
 
@@ -91,16 +94,16 @@ STRIDED = """    {{
     }}
 """
 XPASS = """    {{
-        constexpr int P = {P}, SPLIT = {SPLIT};
+        constexpr int P = {P}, SPLIT = {SPLIT}, TWB = {TWB};
         using XC = XCfg<N1, N2, P, SPLIT>;
         XKern k;
-        k.P = P; k.split = SPLIT; k.threads = XC::THREADS; k.lds = XC::LDS_BYTES;
-        k.fn[XM_ORDER0] = (const void*)&k_xpass<N1, N2, P, XM_ORDER0, SPLIT>;
-        k.fn[XM_MID] = (const void*)&k_xpass<N1, N2, P, XM_MID, SPLIT>;
-        k.fn[XM_LAST] = (const void*)&k_xpass<N1, N2, P, XM_LAST, SPLIT>;
-        k.fn[XM_FWD] = (const void*)&k_xpass<N1, N2, P, XM_FWD, SPLIT>;
-        k.fn[XM_INV] = (const void*)&k_xpass<N1, N2, P, XM_INV, SPLIT>;
-        k.fn[XM_LASTA] = (const void*)&k_xpass<N1, N2, P, XM_LASTA, SPLIT>;
+        k.P = P; k.split = SPLIT; k.twb = TWB; k.threads = XC::THREADS; k.lds = XC::LDS_BYTES;
+        k.fn[XM_ORDER0] = (const void*)&k_xpass<N1, N2, P, XM_ORDER0, SPLIT, TWB>;
+        k.fn[XM_MID] = (const void*)&k_xpass<N1, N2, P, XM_MID, SPLIT, TWB>;
+        k.fn[XM_LAST] = (const void*)&k_xpass<N1, N2, P, XM_LAST, SPLIT, TWB>;
+        k.fn[XM_FWD] = (const void*)&k_xpass<N1, N2, P, XM_FWD, SPLIT, TWB>;
+        k.fn[XM_INV] = (const void*)&k_xpass<N1, N2, P, XM_INV, SPLIT, TWB>;
+        k.fn[XM_LASTA] = (const void*)&k_xpass<N1, N2, P, XM_LASTA, SPLIT, TWB>;
         e.xk[e.nx++] = k;
     }}
 """
@@ -113,8 +116,8 @@ def instance_source(n, n1, n2):
     src = HEAD.format(N=n, N1=n1, N2=n2)
     for t, split in strided_variants(n1, n2):
         src += STRIDED.format(T=t, SPLIT=split)
-    for p, split in x_variants(n1, n2):
-        src += XPASS.format(P=p, SPLIT=split)
+    for p, split, twb in x_variants(n1, n2):
+        src += XPASS.format(P=p, SPLIT=split, TWB=twb)
     return src + TAIL.format()
 
 

@@ -1045,6 +1045,13 @@ int ose_set_autotune(ose_ctx* c, int on)
     return OSE_OK;
 }
 
+// x-pass launch-shape code: P + 1000 twb, negative for the split exchange
+static int xcode(const XKern& k)
+{
+    const int v = k.P + 1000 * k.twb;
+    return k.split ? -v : v;
+}
+
 int ose_get_tuning(ose_ctx* c, int model_id, int* shape)
 {
     if (!c || !shape) return OSE_EINVAL;
@@ -1054,7 +1061,7 @@ int ose_get_tuning(ose_ctx* c, int model_id, int* shape)
     shape[0] = code(m.ey->yf[m.sel_yf]);
     shape[1] = code(m.ez->zs[m.sel_z]);
     shape[2] = code(m.ey->yi[m.sel_yi]);
-    shape[3] = m.ex->xk[m.sel_x].split ? -m.ex->xk[m.sel_x].P : m.ex->xk[m.sel_x].P;
+    shape[3] = xcode(m.ex->xk[m.sel_x]);
     return OSE_OK;
 }
 
@@ -1076,7 +1083,7 @@ int ose_set_tuning(ose_ctx* c, int model_id, const int* shape)
     const int yi = find_strided(m.ey, 2, shape[2]);
     int x = -1;
     for (int v = 0; v < m.ex->nx; ++v)
-        if ((m.ex->xk[v].split ? -m.ex->xk[v].P : m.ex->xk[v].P) == shape[3]) x = v;
+        if (xcode(m.ex->xk[v]) == shape[3]) x = v;
     if (yf < 0 || z < 0 || yi < 0 || x < 0)
         return c->fail(OSE_EINVAL, "ose_set_tuning: launch shape {%d, %d, %d, %d} is not compiled for %dx%dx%d",
                        shape[0], shape[1], shape[2], shape[3], m.d.nxb, m.d.nyb, m.d.nzb);
@@ -1108,7 +1115,7 @@ int ose_launch_shapes(ose_ctx* c, int model_id, int* strided, int* nstrided, int
     *nstrided = n;
     n = 0;
     for (int v = 0; v < m.ex->nx; ++v) {
-        if (xshapes && n < *nxs) xshapes[n] = m.ex->xk[v].split ? -m.ex->xk[v].P : m.ex->xk[v].P;
+        if (xshapes && n < *nxs) xshapes[n] = xcode(m.ex->xk[v]);
         ++n;
     }
     *nxs = n;

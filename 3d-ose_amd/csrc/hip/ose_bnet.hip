@@ -173,22 +173,51 @@ __global__ __launch_bounds__(256) void k_bn_conv(ConvArgs a)
     const long ntiles = (a.sout.count + 15) / 16;
     const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
     const uint16_t* in = a.in + (long)shot * a.in_shot;
+    const bool fast_ok = a.sin.tz == a.sout.tz + 2 && a.sin.ty == a.sout.ty + 2 && a.sin.tx == a.sout.tx + 2;
     for (long tile = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); tile < ntiles; tile += nwaves) {
         const long cell = tile * 16 + r16;
         const bool valid = cell < a.sout.count;
+        NbFast nf{0u, 0, 0, false};
         int x = 0, y = 0, z = 0;
-        if (valid) shell_coords(a.sout, cell, x, y, z);
+        if (valid) {
+            nf = nb_fast(a.sout, a.sin, (unsigned)cell, fast_ok);
+            if (!nf.fast) shell_coords(a.sout, cell, x, y, z);
+        }
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        if (__all(nf.fast || !valid)) {
+            // the whole tile on constant offsets: per dz plane, nine loads
+            // are issued together, then nine MFMAs
+            const unsigned base = valid ? nf.base : 0u;
 #pragma unroll
-        for (int tap = 0; tap < BN_TAPS; ++tap) {
-            const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-            bf16x4 av = {0, 0, 0, 0};
-            if (valid) {
-                const long nb = shell_index(a.sin, wrapc(x + dx, a.sin.nx), wrapc(y + dy, a.sin.ny),
-                                            wrapc(z + dz, a.sin.nz));
-                if (nb >= 0) av = *(const bf16x4*)(in + nb * 16 + 4 * g);
+            for (int c = 0; c < 3; ++c) {
+                bf16x4 av[9];
+#pragma unroll
+                for (int k = 0; k < 9; ++k) {
+                    const int tap = 9 * c + k;
+                    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+                    const unsigned nb = valid ? base + (unsigned)(dz * nf.oz + dy * nf.oy + dx) : 0u;
+                    av[k] = *(const bf16x4*)(in + (size_t)nb * 16 + 4 * g);
+                    if (!valid) av[k] = bf16x4{0, 0, 0, 0};
+                }
+#pragma unroll
+                for (int k = 0; k < 9; ++k) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[k], wb[9 * c + k], acc, 0, 0, 0);
             }
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, wb[tap], acc, 0, 0, 0);
+        } else {
+#pragma unroll
+            for (int tap = 0; tap < BN_TAPS; ++tap) {
+                const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+                bf16x4 av = {0, 0, 0, 0};
+                if (valid) {
+                    long nb;
+                    if (nf.fast)
+                        nb = (long)(nf.base + (unsigned)(dz * nf.oz + dy * nf.oy + dx));
+                    else
+                        nb = shell_index(a.sin, wrapc(x + dx, a.sin.nx), wrapc(y + dy, a.sin.ny),
+                                         wrapc(z + dz, a.sin.nz));
+                    if (nb >= 0) av = *(const bf16x4*)(in + nb * 16 + 4 * g);
+                }
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, wb[tap], acc, 0, 0, 0);
+            }
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {

@@ -13,9 +13,10 @@ struct StridedKern {
 };
 
 // one compiled launch shape of the x pass: P lines per workgroup, a kernel
-// per XMode, split = 1 for the half-LDS exchange
+// per XMode, split = 1 for the half-LDS exchange, twb = 1 for the
+// scheduling barrier before the twiddle multiplies
 struct XKern {
-    int P = 0, split = 0, threads = 0;
+    int P = 0, split = 0, twb = 0, threads = 0;
     size_t lds = 0;
     const void* fn[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 };

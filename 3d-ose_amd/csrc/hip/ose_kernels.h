@@ -298,7 +298,10 @@ __device__ __forceinline__ void line_exchange(const float2* src, bool wr, float2
 
 // inverse x transform of one line: thread t < N1 loads X[t + N1 m2]; on
 // return thread t < N2 holds x[N2 n1 + t] (unnormalised) in res[n1].
-template <int N1, int N2, int PN, int SPLIT>
+// TWB = 1 places a scheduling barrier before the twiddle multiply, so the
+// twiddle loads are not hoisted over the register DFT (lower peak register
+// use for some extents, at the price of exposed load latency).
+template <int N1, int N2, int PN, int SPLIT, int TWB>
 __device__ __forceinline__ void x_inverse(const float2* __restrict__ src, bool okl, int t,
                                           void* L, const float2* __restrict__ tw,
                                           float2* __restrict__ w, float2* __restrict__ res)
@@ -307,6 +310,7 @@ __device__ __forceinline__ void x_inverse(const float2* __restrict__ src, bool o
 #pragma unroll
         for (int m2 = 0; m2 < N2; ++m2) w[m2] = okl ? src[t + N1 * m2] : make_float2(0.f, 0.f);
         rdft<N2, 1>(w);
+        if (TWB) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int n2 = 1; n2 < N2; ++n2) w[n2] = cmulc(w[n2], tw[t * n2]);
     }
@@ -318,13 +322,14 @@ __device__ __forceinline__ void x_inverse(const float2* __restrict__ src, bool o
 
 // forward x transform: thread t < N2 holds x[N2 n1 + t] in u[n1]; writes
 // X[k] to dst (natural order).
-template <int N1, int N2, int PN, int SPLIT>
+template <int N1, int N2, int PN, int SPLIT, int TWB>
 __device__ __forceinline__ void x_forward(float2* __restrict__ u, bool okl, int t,
                                           void* L, const float2* __restrict__ tw,
                                           float2* __restrict__ w, float2* dst)
 {
     if (t < N2) {
         rdft<N1, -1>(u);
+        if (TWB) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int k1 = 1; k1 < N1; ++k1) u[k1] = cmul(u[k1], tw[t * k1]);
     }
@@ -339,7 +344,7 @@ __device__ __forceinline__ void x_forward(float2* __restrict__ u, bool okl, int 
     }
 }
 
-template <int N1, int N2, int P, int MODE, int SPLIT>
+template <int N1, int N2, int P, int MODE, int SPLIT, int TWB>
 __global__ __launch_bounds__((XCfg<N1, N2, P, SPLIT>::THREADS))
 void k_xpass(XArgs a)
 {
@@ -364,11 +369,11 @@ void k_xpass(XArgs a)
 #pragma unroll
             for (int n1 = 0; n1 < N1; ++n1) u[n1] = a.A[lbase + N2 * n1 + t];
         }
-        x_forward<N1, N2, C::PN, SPLIT>(u, okl, t, L, a.tw, w, a.Xout + lbase);
+        x_forward<N1, N2, C::PN, SPLIT, TWB>(u, okl, t, L, a.tw, w, a.Xout + lbase);
         return;
     }
     if (MODE == XM_INV) {
-        x_inverse<N1, N2, C::PN, SPLIT>(a.Xprev + lbase, okl, t, L, a.tw, w, u);
+        x_inverse<N1, N2, C::PN, SPLIT, TWB>(a.Xprev + lbase, okl, t, L, a.tw, w, u);
         if (t < N2 && okl) {
 #pragma unroll
             for (int n1 = 0; n1 < N1; ++n1)
@@ -378,7 +383,7 @@ void k_xpass(XArgs a)
     }
 
     // L W_j / (v/R) in physical x (unnormalised inverse; the symbol carried 1/N)
-    x_inverse<N1, N2, C::PN, SPLIT>(a.S + lbase, okl, t, L, a.tw, w, u);
+    x_inverse<N1, N2, C::PN, SPLIT, TWB>(a.S + lbase, okl, t, L, a.tw, w, u);
 
     if (t < N2 && okl) {
         const float gyz = a.gy[y] * a.gz[z];
@@ -429,7 +434,7 @@ void k_xpass(XArgs a)
             }
         }
     }
-    if (MODE != XM_LASTA) x_forward<N1, N2, C::PN, SPLIT>(u, okl, t, L, a.tw, w, a.Xout + lbase);
+    if (MODE != XM_LASTA) x_forward<N1, N2, C::PN, SPLIT, TWB>(u, okl, t, L, a.tw, w, a.Xout + lbase);
 }
 
 // ---------------------------------------------------------------------

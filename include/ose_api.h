@@ -127,8 +127,10 @@ int ose_reset_stats(ose_ctx* ctx);
  * strided passes, lines per workgroup for the x pass) and the fastest kept.
  * No counterpart in the reference (cuFFT chose its own plan, ose_3d.cu:226). */
 int ose_set_autotune(ose_ctx* ctx, int on);
-/* shape[4] = {T forward-y, T z-symbol, T inverse-y, P x-pass} of a model;
- * a negative value marks the half-LDS (split real/imaginary) exchange.    */
+/* shape[4] = {T forward-y, T z-symbol, T inverse-y, x-pass code} of a
+ * model; x-pass code = P + 1000 * b where b = 1 marks the variant with a
+ * scheduling barrier before the twiddle multiplies; a negative value marks
+ * the half-LDS (split real/imaginary) exchange.                           */
 int ose_get_tuning(ose_ctx* ctx, int model_id, int* shape);
 /* Pin a model's launch shapes (same encoding as ose_get_tuning; the
  * strided entries must be compiled for the y and z extents).  Skips the

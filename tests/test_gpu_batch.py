@@ -84,7 +84,7 @@ def test_batched_shots_and_models_match_oracle(autotune):
     assert max(e[3] for e in errs) < 1e-4
 
 
-@pytest.mark.parametrize("padded", [(48, 336, 336), (336, 48, 48)])
+@pytest.mark.parametrize("padded", [(48, 336, 336), (336, 48, 48), (600, 32, 24)])
 def test_every_launch_shape_gives_the_same_step(padded):
     """Every compiled launch shape (T columns / P lines per workgroup, whole
     or split LDS exchange) runs the same arithmetic: one step from the same
