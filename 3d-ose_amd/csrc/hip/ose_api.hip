@@ -317,7 +317,7 @@ int collect(ose_ctx* c)
         HIPCHK(c, hipEventElapsedTime(&ms, p.e0, p.e1));
         c->st_ms[p.kind] += ms;
         c->st_n[p.kind] += 1;
-        c->st_bytes[p.kind] = p.bytes;
+        c->st_bytes[p.kind] += p.bytes;
         c->ev_pool.push_back(p.e0);
         c->ev_pool.push_back(p.e1);
     }
@@ -1127,7 +1127,7 @@ int ose_pass_stats(ose_ctx* c, int pass, double* avg_ms, long* launches, double*
     if (!c || pass < 0 || pass >= PK_N) return OSE_EINVAL;
     if (avg_ms) *avg_ms = c->st_n[pass] ? c->st_ms[pass] / c->st_n[pass] : 0.0;
     if (launches) *launches = c->st_n[pass];
-    if (bytes) *bytes = c->st_bytes[pass];
+    if (bytes) *bytes = c->st_n[pass] ? c->st_bytes[pass] / c->st_n[pass] : 0.0;
     return OSE_OK;
 }
 

@@ -205,7 +205,8 @@ def train(seeds=range(1000, 1016), n=64, thin=8, wide=64, iters=4000, lr=2e-3, l
     return p, losses
 
 
-def evaluate(params, seeds, n=64, thin=8, wide=64, nt=400, fm=20.0, model_kind="train", guard=4.0, replace=True):
+def evaluate(params, seeds, n=64, thin=8, wide=64, nt=400, fm=20.0, model_kind="train", guard=4.0, replace=True,
+             log=None):
     """Rel-L2 of surface seismograms and reflected energy of thin runs
     (plain taper, learned pad) against the wide-sponge run."""
     rows = []
@@ -237,7 +238,10 @@ def evaluate(params, seeds, n=64, thin=8, wide=64, nt=400, fm=20.0, model_kind="
             row[f"{name}_rel_l2"] = float(np.linalg.norm(res[name]["tr"] - ref_tr) / np.linalg.norm(ref_tr))
             row[f"{name}_reflected"] = float(np.linalg.norm(res[name]["u"] - ref_u) ** 2 / np.linalg.norm(ref_u) ** 2)
         row["learned_status"] = res["learned"]["status"]
+        row["run_seconds"] = {k: round(v["s"], 3) for k, v in res.items()}
         rows.append(row)
+        if log:
+            log(json.dumps(row))
     return rows
 
 
@@ -266,10 +270,8 @@ def main():
         a, b = (int(x) for x in args.seeds.split(":"))
         p = np.load(args.weights)
         assert p.size == boundary_net_size()
-        rows = evaluate(p, range(a, b), n=args.n, thin=args.thin, wide=args.wide, nt=args.nt, guard=args.guard,
-                        replace=not args.keep_taper)
-        for r in rows:
-            print(json.dumps(r))
+        evaluate(p, range(a, b), n=args.n, thin=args.thin, wide=args.wide, nt=args.nt, guard=args.guard,
+                 replace=not args.keep_taper, log=lambda m: print(m, flush=True))
 
 
 if __name__ == "__main__":

@@ -118,7 +118,9 @@ int ose_set_options(ose_ctx* ctx, int use_graph, int profile);
 /* Per-pass statistics since the last reset: pass 0 forward-y, 1 z-symbol,
  * 2 inverse-y, 3 x-epilogue (middle orders), 4 x-epilogue (first order),
  * 5 x-epilogue (last order), 6 injection.  avg_ms is the mean HIP-event
- * duration of one launch, bytes the algorithmic HBM bytes of one launch.   */
+ * duration of one launch, bytes the mean algorithmic HBM bytes of one
+ * launch (launches of several models differ in size), so bytes / avg_ms
+ * is the pass's aggregate rate.                                             */
 int ose_pass_stats(ose_ctx* ctx, int pass, double* avg_ms, long* launches, double* bytes);
 int ose_reset_stats(ose_ctx* ctx);
 
