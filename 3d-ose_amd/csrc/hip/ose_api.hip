@@ -353,6 +353,7 @@ int launch_x(ose_ctx* c, int kind, const XKern& k, int mode, XArgs& a, int nline
              double bytes)
 {
     a.nlines = nlines;
+    if (!a.twf) a.twf = a.tw;
     // every pointer the mode dereferences must be set before launching
     bool ok = a.tw != nullptr && k.fn[mode] != nullptr;
     switch (mode) {

@@ -228,7 +228,11 @@ struct XArgs {
     const float* gx;
     const float* gy;
     const float* gz;
-    const float2* tw;       // x root table
+    const float2* tw;       // x root table (inverse transform)
+    const float2* twf;      // the same table for the forward transform: a
+                            // separate pointer keeps the compiler from holding
+                            // the inverse's twiddle loads in registers for reuse
+                            // by the forward (register pressure, occupancy)
     long shot_stride;
     int ny;
     int nlines;             // ny * nz
@@ -369,7 +373,7 @@ void k_xpass(XArgs a)
 #pragma unroll
             for (int n1 = 0; n1 < N1; ++n1) u[n1] = a.A[lbase + N2 * n1 + t];
         }
-        x_forward<N1, N2, C::PN, SPLIT, TWB>(u, okl, t, L, a.tw, w, a.Xout + lbase);
+        x_forward<N1, N2, C::PN, SPLIT, TWB>(u, okl, t, L, a.twf, w, a.Xout + lbase);
         return;
     }
     if (MODE == XM_INV) {
@@ -434,7 +438,7 @@ void k_xpass(XArgs a)
             }
         }
     }
-    if (MODE != XM_LASTA) x_forward<N1, N2, C::PN, SPLIT, TWB>(u, okl, t, L, a.tw, w, a.Xout + lbase);
+    if (MODE != XM_LASTA) x_forward<N1, N2, C::PN, SPLIT, TWB>(u, okl, t, L, a.twf, w, a.Xout + lbase);
 }
 
 // ---------------------------------------------------------------------

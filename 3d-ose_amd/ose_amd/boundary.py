@@ -216,7 +216,10 @@ def evaluate(params, seeds, n=64, thin=8, wide=64, nt=400, fm=20.0, model_kind="
         s, h = _wavelet(nt, fm, wide_g.dt)
         src = (n // 2, n // 2, 2)
         res = {}
-        for name, g, p in (("wide", wide_g, None), ("plain", thin_g, None), ("learned", thin_g, params)):
+        runs = [("wide", wide_g, None), ("plain", thin_g, None)]
+        if params is not None:
+            runs.append(("learned", thin_g, params))
+        for name, g, p in runs:
             with Context(0) as c:
                 m = g.add_to(c)
                 sid = c.add_shot(m, (g.origin[0] + src[0], g.origin[1] + src[1], g.origin[2] + src[2]), s, h)
@@ -234,10 +237,12 @@ def evaluate(params, seeds, n=64, thin=8, wide=64, nt=400, fm=20.0, model_kind="
             res[name] = dict(tr=tr, u=u_int, s=dt, status=st)
         ref_tr, ref_u = res["wide"]["tr"], res["wide"]["u"]
         row = {"seed": int(seed)}
-        for name in ("plain", "learned"):
+        row["thin"] = thin
+        for name in [r[0] for r in runs[1:]]:
             row[f"{name}_rel_l2"] = float(np.linalg.norm(res[name]["tr"] - ref_tr) / np.linalg.norm(ref_tr))
             row[f"{name}_reflected"] = float(np.linalg.norm(res[name]["u"] - ref_u) ** 2 / np.linalg.norm(ref_u) ** 2)
-        row["learned_status"] = res["learned"]["status"]
+        if params is not None:
+            row["learned_status"] = res["learned"]["status"]
         row["run_seconds"] = {k: round(v["s"], 3) for k, v in res.items()}
         rows.append(row)
         if log:
@@ -262,14 +267,17 @@ def main():
     ap.add_argument("--guard", type=float, default=4.0)
     ap.add_argument("--keep-taper", action="store_true", help="the network acts on top of the thin taper")
     ap.add_argument("--rounds", type=int, default=3, help="rounds of rollout pair aggregation")
+    ap.add_argument("--plain-only", action="store_true", help="eval: wide sponge and thin taper only")
     args = ap.parse_args()
     if args.cmd == "train":
         train(range(1000, 1000 + args.models), n=args.n, thin=args.thin, wide=args.wide, iters=args.iters,
               lr=args.lr, out=args.out, replace=not args.keep_taper, rounds=args.rounds)
     else:
         a, b = (int(x) for x in args.seeds.split(":"))
-        p = np.load(args.weights)
-        assert p.size == boundary_net_size()
+        p = None
+        if not args.plain_only:
+            p = np.load(args.weights)
+            assert p.size == boundary_net_size()
         evaluate(p, range(a, b), n=args.n, thin=args.thin, wide=args.wide, nt=args.nt, guard=args.guard,
                  replace=not args.keep_taper, log=lambda m: print(m, flush=True))
 

@@ -45,3 +45,14 @@ def test_host_only_entry_points():
     assert L.ose_supported_extent(141) == 144
     assert L.ose_add_model(None, None, None, None) == _lib.OSE_EINVAL
     assert L.ose_run(None, 1) == _lib.OSE_EINVAL
+
+
+def test_create_without_device_reports_edevice():
+    """No visible GPU: ose_create fails with OSE_EDEVICE instead of exiting."""
+    from ose_amd import _lib
+    L = _lib.hip()
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is visible here")
+    ctx = ctypes.c_void_p()
+    assert L.ose_create(ctypes.byref(ctx), 0) == _lib.OSE_EDEVICE
+    assert not ctx.value
