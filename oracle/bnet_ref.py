@@ -2,7 +2,7 @@
 
 TEST INFRASTRUCTURE ONLY (the checker of the HIP kernels in
 csrc/hip/ose_bnet.hip).  The reference (Littleke/3D-OSE) has no learned
-boundary; this module defines the network the north star asks for so that
+boundary; this module defines the network the learned thin pad uses so that
 the GPU forward pass and gradients can be checked against FP64.
 
 Grid: padded [nzb][nyb][nxb], interior box origin (bl, bf, bu), size

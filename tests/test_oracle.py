@@ -112,7 +112,7 @@ def _plane_wave_case(K=None, coef_hook=None):
 
 def test_jacobi_anger_plane_wave():
     """Constant v, no taper: one step of a Fourier mode is exp(-i dt v|k|)
-    times the mode, to within the series truncation tail (SURVEY 0.1)."""
+    times the mode, to within the series truncation tail."""
     err, tail = _plane_wave_case()
     assert err <= tail + 1e-12
     err20, tail20 = _plane_wave_case(K=20)
