@@ -62,8 +62,10 @@ def x_variants(n1, n2):
     nt = max(n1, n2)
     pn = n1 | 1
     out = []
-    for twb in ((0, 1) if n1 * n2 >= 600 else (0,)):
-        for p in sorted({max(1, 128 // nt), max(1, 256 // nt)}):
+    big = n1 * n2 >= 600
+    ps = {max(1, 128 // nt), max(1, 256 // nt)} | ({max(1, 64 // nt)} if big else set())
+    for twb in ((0, 1) if big else (0,)):
+        for p in sorted(ps):
             full = 8 * p * n2 * pn
             if p * nt <= 1024 and full <= LDS_MAX:
                 out.append((p, 0, twb))

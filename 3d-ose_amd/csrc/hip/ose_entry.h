@@ -21,7 +21,7 @@ struct XKern {
     const void* fn[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 
-constexpr int kMaxVariants = 8;
+constexpr int kMaxVariants = 12;
 
 struct FftEntry {
     int N = 0, N1 = 0, N2 = 0;
