@@ -177,15 +177,22 @@ def generate_rollout_pairs(net, params, seeds, n=64, thin=8, wide=64, nt=400, t0
 
 
 def train(seeds=range(1000, 1016), n=64, thin=8, wide=64, iters=4000, lr=2e-3, log=print, out=None,
-          replace=True, rounds=3):
+          replace=True, rounds=3, init=None):
+    """init: parameters to continue from; training then starts with a
+    rollout round (no teacher-forced pairs)."""
     vi = H.model("train", n, n, n, seed=int(seeds[0]))
     thin_g, _ = make_pair_grids(vi, thin, wide, thin_taper="none")
     net = BoundaryNet(thin_g.v, thin_g.origin, thin_g.interior)
-    t = time.time()
-    generate_pairs(net, seeds, n=n, thin=thin, wide=wide, replace=replace, log=log)
-    log(f"[boundary] pairs generated in {time.time() - t:.1f} s")
+    first = 0
+    if init is not None:
+        net.set_params(init)
+        first = 1
+    else:
+        t = time.time()
+        generate_pairs(net, seeds, n=n, thin=thin, wide=wide, replace=replace, log=log)
+        log(f"[boundary] pairs generated in {time.time() - t:.1f} s")
     losses = []
-    for r in range(rounds if replace else 1):
+    for r in range(first, (first + rounds) if replace else 1):
         if r > 0:
             t = time.time()
             generate_rollout_pairs(net, net.params(), seeds, n=n, thin=thin, wide=wide, replace=replace, log=log)
@@ -268,10 +275,13 @@ def main():
     ap.add_argument("--keep-taper", action="store_true", help="the network acts on top of the thin taper")
     ap.add_argument("--rounds", type=int, default=3, help="rounds of rollout pair aggregation")
     ap.add_argument("--plain-only", action="store_true", help="eval: wide sponge and thin taper only")
+    ap.add_argument("--init", default=None, help="train: parameters to continue from")
+    ap.add_argument("--seed0", type=int, default=1000, help="train: first model seed")
     args = ap.parse_args()
     if args.cmd == "train":
-        train(range(1000, 1000 + args.models), n=args.n, thin=args.thin, wide=args.wide, iters=args.iters,
-              lr=args.lr, out=args.out, replace=not args.keep_taper, rounds=args.rounds)
+        init = np.load(args.init) if args.init else None
+        train(range(args.seed0, args.seed0 + args.models), n=args.n, thin=args.thin, wide=args.wide, iters=args.iters,
+              lr=args.lr, out=args.out, replace=not args.keep_taper, rounds=args.rounds, init=init)
     else:
         a, b = (int(x) for x in args.seeds.split(":"))
         p = None
