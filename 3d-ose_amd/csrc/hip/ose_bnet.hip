@@ -159,16 +159,37 @@ struct ConvArgs {
     long y_shot;
 };
 
+// Two taps per MFMA (16x16x32, K = 2 taps x 16 channels): lane group g
+// (lanes 16g..16g+15) supplies channels 8(g%2)..8(g%2)+7 of tap 2p + g/2 for
+// the A operand (one 16-byte load) and the matching weights for B.  Tap 27
+// does not exist: its weights are zero.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+constexpr int BN_PAIRS = (BN_TAPS + 1) / 2;
+
+__device__ __forceinline__ int tap_offset(int tap, int oy, int oz)
+{
+    return (tap / 9 - 1) * oz + ((tap / 3) % 3 - 1) * oy + (tap % 3 - 1);
+}
+
 template <int EPI>
 __global__ __launch_bounds__(256) void k_bn_conv(ConvArgs a)
 {
     const int lane = threadIdx.x & 63;
     const int g = lane >> 4;
     const int r16 = lane & 15;
+    const int half = g >> 1;                 // which tap of the pair
+    const int ch0 = 8 * (g & 1);             // first channel of this lane
     const int shot = blockIdx.y;
-    bf16x4 wb[BN_TAPS];
+    s16x8 wb[BN_PAIRS];
 #pragma unroll
-    for (int tap = 0; tap < BN_TAPS; ++tap) wb[tap] = *(const bf16x4*)(a.W + (tap * 16 + r16) * 16 + 4 * g);
+    for (int p = 0; p < BN_PAIRS; ++p) {
+        const int tap = 2 * p + half;
+        if (tap < BN_TAPS)
+            wb[p] = *(const s16x8*)(a.W + (tap * 16 + r16) * 16 + ch0);
+        else
+            wb[p] = s16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    }
     const float bias = a.bias[r16];
     const long ntiles = (a.sout.count + 15) / 16;
     const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
@@ -185,38 +206,44 @@ __global__ __launch_bounds__(256) void k_bn_conv(ConvArgs a)
         }
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
         if (__all(nf.fast || !valid)) {
-            // the whole tile on constant offsets: per dz plane, nine loads
-            // are issued together, then nine MFMAs
+            // the whole tile on constant offsets: seven pair-loads are issued
+            // together, then seven MFMAs
             const unsigned base = valid ? nf.base : 0u;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                bf16x4 av[9];
+            for (int c = 0; c < 2; ++c) {
+                s16x8 av[7];
 #pragma unroll
-                for (int k = 0; k < 9; ++k) {
-                    const int tap = 9 * c + k;
-                    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-                    const unsigned nb = valid ? base + (unsigned)(dz * nf.oz + dy * nf.oy + dx) : 0u;
-                    av[k] = *(const bf16x4*)(in + (size_t)nb * 16 + 4 * g);
-                    if (!valid) av[k] = bf16x4{0, 0, 0, 0};
+                for (int k = 0; k < 7; ++k) {
+                    const int p = 7 * c + k;
+                    const int tap = 2 * p + half;
+                    const bool ok = valid && tap < BN_TAPS;
+                    const unsigned nb = ok ? base + (unsigned)tap_offset(tap, nf.oy, nf.oz) : 0u;
+                    av[k] = *(const s16x8*)(in + (size_t)nb * 16 + ch0);
+                    if (!ok) av[k] = s16x8{0, 0, 0, 0, 0, 0, 0, 0};
                 }
 #pragma unroll
-                for (int k = 0; k < 9; ++k) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[k], wb[9 * c + k], acc, 0, 0, 0);
+                for (int k = 0; k < 7; ++k)
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av[k]),
+                                                                  __builtin_bit_cast(bf16x8, wb[7 * c + k]), acc, 0,
+                                                                  0, 0);
             }
         } else {
 #pragma unroll
-            for (int tap = 0; tap < BN_TAPS; ++tap) {
-                const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-                bf16x4 av = {0, 0, 0, 0};
-                if (valid) {
+            for (int p = 0; p < BN_PAIRS; ++p) {
+                const int tap = 2 * p + half;
+                s16x8 av = {0, 0, 0, 0, 0, 0, 0, 0};
+                if (valid && tap < BN_TAPS) {
+                    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
                     long nb;
                     if (nf.fast)
                         nb = (long)(nf.base + (unsigned)(dz * nf.oz + dy * nf.oy + dx));
                     else
                         nb = shell_index(a.sin, wrapc(x + dx, a.sin.nx), wrapc(y + dy, a.sin.ny),
                                          wrapc(z + dz, a.sin.nz));
-                    if (nb >= 0) av = *(const bf16x4*)(in + nb * 16 + 4 * g);
+                    if (nb >= 0) av = *(const s16x8*)(in + nb * 16 + ch0);
                 }
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, wb[tap], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av),
+                                                              __builtin_bit_cast(bf16x8, wb[p]), acc, 0, 0, 0);
             }
         }
 #pragma unroll
