@@ -160,8 +160,8 @@ def learned_leg(params, steps, warmup, plain_value, weights):
         "network_ms_per_step": lr["ms_per_step"] - res["thin_taper"]["ms_per_step"],
         "weights": os.path.relpath(weights, ROOT),
         "accuracy": ("not at the rel-L2 1e-3 bar: on held-out 256^3 models over 2000 steps the trained 8-cell "
-                     "pad leaves 5.5-8.8x less reflected interior energy than the 8-cell taper; seismogram rel-L2 "
-                     "against a 64-cell sponge 0.51-0.52 (taper 0.60-0.72); profiles/learned_pad_eval_256cube_2000steps.txt"),
+                     "pad leaves 5.4-9.9x less reflected interior energy than the 8-cell taper; seismogram rel-L2 "
+                     "against a 64-cell sponge 0.52-0.53 (taper 0.60-0.72); profiles/learned_pad_eval_256cube_2000steps.txt"),
     }
 
 
