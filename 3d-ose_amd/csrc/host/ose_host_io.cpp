@@ -1,4 +1,5 @@
-// ose_host_io.cpp -- seismogram (raw and SU) and snapshot files.
+// ose_host_io.cpp -- seismogram (raw, SU, SEG-Y) and snapshot files.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -20,6 +21,90 @@ T get(const unsigned char* h, int off)
     T v;
     std::memcpy(&v, h + off, sizeof(T));
     return v;
+}
+
+// big-endian fields of SEG-Y files
+void put_be32(unsigned char* h, int off, uint32_t v)
+{
+    h[off] = (unsigned char)(v >> 24);
+    h[off + 1] = (unsigned char)(v >> 16);
+    h[off + 2] = (unsigned char)(v >> 8);
+    h[off + 3] = (unsigned char)v;
+}
+void put_be16(unsigned char* h, int off, uint16_t v)
+{
+    h[off] = (unsigned char)(v >> 8);
+    h[off + 1] = (unsigned char)v;
+}
+uint32_t get_be32(const unsigned char* h, int off)
+{
+    return ((uint32_t)h[off] << 24) | ((uint32_t)h[off + 1] << 16) | ((uint32_t)h[off + 2] << 8) | h[off + 3];
+}
+uint16_t get_be16(const unsigned char* h, int off) { return (uint16_t)((h[off] << 8) | h[off + 1]); }
+
+// The 240-byte trace header shared by SU (native byte order) and SEG-Y
+// (big-endian); byte offsets of the SEG-Y rev 1 standard.
+template <bool BE>
+void trace_header(unsigned char* h, int r, int nt, double us, double gx, double gy, double src_x, double src_y,
+                  int shot_index)
+{
+    std::memset(h, 0, 240);
+    auto i32 = [h](int off, int32_t v) {
+        if (BE) put_be32(h, off, (uint32_t)v);
+        else put<int32_t>(h, off, v);
+    };
+    auto i16 = [h](int off, int16_t v) {
+        if (BE) put_be16(h, off, (uint16_t)v);
+        else put<int16_t>(h, off, v);
+    };
+    i32(0, r + 1);                                 // tracl
+    i32(4, r + 1);                                 // tracr
+    i32(8, shot_index + 1);                        // fldr
+    i32(12, r + 1);                                // tracf
+    i32(16, shot_index + 1);                       // ep
+    i32(20, r + 1);                                // cdp
+    i16(28, 1);                                    // trid: seismic data
+    i32(36, (int32_t)std::lround(std::hypot(gx - src_x, gy - src_y)));   // offset
+    i16(70, (int16_t)-100);                        // scalco: coordinates / 100
+    i32(72, (int32_t)std::lround(src_x * 100.0));
+    i32(76, (int32_t)std::lround(src_y * 100.0));
+    i32(80, (int32_t)std::lround(gx * 100.0));
+    i32(84, (int32_t)std::lround(gy * 100.0));
+    i16(88, 1);                                    // counit: length (m)
+    i16(114, (int16_t)(uint16_t)nt);               // ns
+    i16(116, (int16_t)(uint16_t)std::lround(us));  // dt (us)
+}
+
+// ASCII -> EBCDIC (code page 037) for the characters of the textual header
+unsigned char ebcdic(char c)
+{
+    if (c >= 'a' && c <= 'z') c = (char)(c - 'a' + 'A');
+    if (c >= '0' && c <= '9') return (unsigned char)(0xF0 + (c - '0'));
+    if (c >= 'A' && c <= 'I') return (unsigned char)(0xC1 + (c - 'A'));
+    if (c >= 'J' && c <= 'R') return (unsigned char)(0xD1 + (c - 'J'));
+    if (c >= 'S' && c <= 'Z') return (unsigned char)(0xE2 + (c - 'S'));
+    switch (c) {
+    case '.': return 0x4B;
+    case '(': return 0x4D;
+    case '+': return 0x4E;
+    case '-': return 0x60;
+    case '/': return 0x61;
+    case ',': return 0x6B;
+    case '_': return 0x6D;
+    case ')': return 0x5D;
+    case ':': return 0x7A;
+    case '=': return 0x7E;
+    default: return 0x40;   // space
+    }
+}
+
+// IBM System/360 single precision -> IEEE
+float ibm_to_ieee(uint32_t b)
+{
+    const double sign = (b >> 31) ? -1.0 : 1.0;
+    const int expo = (int)((b >> 24) & 0x7F) - 64;
+    const double frac = (double)(b & 0x00FFFFFF) / 16777216.0;
+    return (float)(sign * frac * std::pow(16.0, expo));
 }
 
 }  // namespace
@@ -48,24 +133,8 @@ int oseh_write_su(const char* path, const float* traces, int nrec, int nt, doubl
     unsigned char h[240];
     bool ok = true;
     for (int r = 0; r < nrec && ok; ++r) {
-        std::memset(h, 0, sizeof h);
-        put<int32_t>(h, 0, r + 1);                 // tracl
-        put<int32_t>(h, 4, r + 1);                 // tracr
-        put<int32_t>(h, 8, shot_index + 1);        // fldr
-        put<int32_t>(h, 12, r + 1);                // tracf
-        put<int32_t>(h, 16, shot_index + 1);       // ep
-        put<int32_t>(h, 20, r + 1);                // cdp
-        put<int16_t>(h, 28, 1);                    // trid: seismic data
         const double gx = rec_x ? rec_x[r] : 0.0, gy = rec_y ? rec_y[r] : 0.0;
-        put<int32_t>(h, 36, (int32_t)std::lround(std::hypot(gx - src_x, gy - src_y)));   // offset
-        put<int16_t>(h, 70, (int16_t)-100);        // scalco: coordinates / 100
-        put<int32_t>(h, 72, (int32_t)std::lround(src_x * 100.0));
-        put<int32_t>(h, 76, (int32_t)std::lround(src_y * 100.0));
-        put<int32_t>(h, 80, (int32_t)std::lround(gx * 100.0));
-        put<int32_t>(h, 84, (int32_t)std::lround(gy * 100.0));
-        put<int16_t>(h, 88, 1);                    // counit: length (m)
-        put<uint16_t>(h, 114, (uint16_t)nt);       // ns
-        put<uint16_t>(h, 116, (uint16_t)std::lround(us));   // dt (us)
+        trace_header<false>(h, r, nt, us, gx, gy, src_x, src_y, shot_index);
         ok = std::fwrite(h, 1, sizeof h, f) == sizeof h &&
              std::fwrite(traces + (size_t)r * nt, sizeof(float), (size_t)nt, f) == (size_t)nt;
     }
@@ -108,6 +177,112 @@ int oseh_read_su(const char* path, float* traces, size_t cap, int* nrec, int* nt
     std::fclose(f);
     if (nrec) *nrec = n;
     if (nt) *nt = ns < 0 ? 0 : ns;
+    if (dt) *dt = d;
+    return rc;
+}
+
+int oseh_write_segy(const char* path, const float* traces, int nrec, int nt, double dt, const double* rec_x,
+                    const double* rec_y, double src_x, double src_y, int shot_index)
+{
+    if (!path || !traces || nrec < 0 || nt < 0 || nt > 32767) return OSE_EINVAL;
+    const double us = dt * 1e6;
+    if (!(us > 0) || us > 32767.0) return OSE_EINVAL;
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return OSE_EINVAL;
+    // textual header: 40 card images of 80 EBCDIC characters
+    unsigned char text[3200];
+    char line[96];
+    for (int c = 0; c < 40; ++c) {
+        switch (c) {
+        case 0: std::snprintf(line, sizeof line, "C 1 OSE ONE-STEP EXTRAPOLATION SEISMOGRAM, SHOT %d", shot_index + 1); break;
+        case 1: std::snprintf(line, sizeof line, "C 2 TRACES %d SAMPLES %d INTERVAL %.0f US", nrec, nt, us); break;
+        case 2: std::snprintf(line, sizeof line, "C 3 COORDINATES IN METRES TIMES 100 (SCALCO -100)"); break;
+        case 38: std::snprintf(line, sizeof line, "C39 SEG Y REV1"); break;
+        case 39: std::snprintf(line, sizeof line, "C40 END TEXTUAL HEADER"); break;
+        default: std::snprintf(line, sizeof line, "C%2d", c + 1); break;
+        }
+        const size_t n = std::strlen(line);
+        for (int k = 0; k < 80; ++k) text[c * 80 + k] = ebcdic(k < (int)n ? line[k] : ' ');
+    }
+    // binary header (file bytes 3201-3600)
+    unsigned char bin[400];
+    std::memset(bin, 0, sizeof bin);
+    put_be32(bin, 0, 1);                                     // job id
+    put_be32(bin, 4, 1);                                     // line number
+    put_be32(bin, 8, 1);                                     // reel number
+    put_be16(bin, 12, (uint16_t)std::min(nrec, 32767));      // traces per ensemble
+    put_be16(bin, 16, (uint16_t)std::lround(us));            // sample interval
+    put_be16(bin, 18, (uint16_t)std::lround(us));            // original sample interval
+    put_be16(bin, 20, (uint16_t)nt);                         // samples per trace
+    put_be16(bin, 22, (uint16_t)nt);
+    put_be16(bin, 24, 5);                                    // format: 4-byte IEEE float
+    put_be16(bin, 26, 1);                                    // ensemble fold
+    put_be16(bin, 28, 1);                                    // sorting: as recorded
+    put_be16(bin, 54, 1);                                    // measurement system: metres
+    put_be16(bin, 300, 0x0100);                              // SEG Y revision 1.0
+    put_be16(bin, 302, 1);                                   // fixed trace length
+    bool ok = std::fwrite(text, 1, sizeof text, f) == sizeof text && std::fwrite(bin, 1, sizeof bin, f) == sizeof bin;
+    unsigned char h[240];
+    std::vector<unsigned char> data((size_t)nt * 4);
+    for (int r = 0; r < nrec && ok; ++r) {
+        const double gx = rec_x ? rec_x[r] : 0.0, gy = rec_y ? rec_y[r] : 0.0;
+        trace_header<true>(h, r, nt, us, gx, gy, src_x, src_y, shot_index);
+        for (int i = 0; i < nt; ++i) {
+            uint32_t u;
+            std::memcpy(&u, traces + (size_t)r * nt + i, 4);
+            put_be32(data.data(), 4 * i, u);
+        }
+        ok = std::fwrite(h, 1, sizeof h, f) == sizeof h && std::fwrite(data.data(), 1, data.size(), f) == data.size();
+    }
+    const int rc = std::fclose(f);
+    return (ok && rc == 0) ? OSE_OK : OSE_EINVAL;
+}
+
+int oseh_read_segy(const char* path, float* traces, size_t cap, int* nrec, int* nt, double* dt)
+{
+    if (!path) return OSE_EINVAL;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return OSE_EINVAL;
+    unsigned char head[3600];
+    if (std::fread(head, 1, sizeof head, f) != sizeof head) {
+        std::fclose(f);
+        return OSE_EINVAL;
+    }
+    const int fmt = get_be16(head, 3224);
+    const int ns = get_be16(head, 3220);
+    const double d = get_be16(head, 3216) * 1e-6;
+    if ((fmt != 1 && fmt != 5) || ns <= 0) {
+        std::fclose(f);
+        return OSE_EINVAL;
+    }
+    unsigned char h[240];
+    std::vector<unsigned char> buf((size_t)ns * 4);
+    int n = 0, rc = OSE_OK;
+    size_t used = 0;
+    while (std::fread(h, 1, sizeof h, f) == sizeof h) {
+        if (get_be16(h, 114) != ns || std::fread(buf.data(), 1, buf.size(), f) != buf.size()) {
+            rc = OSE_EINVAL;
+            break;
+        }
+        if (traces) {
+            if (used + (size_t)ns > cap) {
+                rc = OSE_EINVAL;
+                break;
+            }
+            for (int i = 0; i < ns; ++i) {
+                const uint32_t u = get_be32(buf.data(), 4 * i);
+                float v;
+                if (fmt == 5) std::memcpy(&v, &u, 4);
+                else v = ibm_to_ieee(u);
+                traces[used + i] = v;
+            }
+        }
+        used += (size_t)ns;
+        ++n;
+    }
+    std::fclose(f);
+    if (nrec) *nrec = n;
+    if (nt) *nt = ns;
     if (dt) *dt = d;
     return rc;
 }

@@ -489,6 +489,9 @@ int oseh_sim_write(oseh_sim* s, const char* dir)
         if (nt <= 65535 && (rc = oseh_write_su((base + ".su").c_str(), tr.data(), nrec, nt, dt, rx.data(),
                                                ry.data(), sx, sy, (int)k)))
             return s->fail(rc, "cannot write " + base + ".su");
+        if (s->geti("segy", 0) && (rc = oseh_write_segy((base + ".segy").c_str(), tr.data(), nrec, nt, dt, rx.data(),
+                                                         ry.data(), sx, sy, (int)k)))
+            return s->fail(rc, "cannot write " + base + ".segy");
     }
     if (s->geti("snap", 0) && (rc = oseh_sim_snapshot(s, dir))) return rc;
     return OSE_OK;

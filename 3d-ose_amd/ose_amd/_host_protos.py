@@ -26,6 +26,8 @@ PROTOS = {
     "oseh_write_raw": ([cp, fp, c_size_t], c_int),
     "oseh_write_su": ([cp, fp, c_int, c_int, c_double, dp, dp, c_double, c_double, c_int], c_int),
     "oseh_read_su": ([cp, fp, c_size_t, ip, ip, dp], c_int),
+    "oseh_write_segy": ([cp, fp, c_int, c_int, c_double, dp, dp, c_double, c_double, c_int], c_int),
+    "oseh_read_segy": ([cp, fp, c_size_t, ip, ip, dp], c_int),
     "oseh_write_snapshot": ([cp, fp, c_int, c_int, c_int], c_int),
     "oseh_sim_create": ([P(vp), cp], c_int),
     "oseh_sim_build": ([vp], c_int),

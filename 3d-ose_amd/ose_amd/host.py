@@ -101,26 +101,41 @@ def model(kind, nx, ny, nz, seed=1, v0=2000.0, v1=3500.0, iface=None):
     return v
 
 
-def write_su(path, traces, dt, rec_x=None, rec_y=None, src=(0.0, 0.0), shot=0):
+def _write_traces(fn, path, traces, dt, rec_x, rec_y, src, shot):
     tr = np.ascontiguousarray(traces, dtype=np.float32)
     nrec, nt = tr.shape
     rx = None if rec_x is None else np.ascontiguousarray(rec_x, dtype=np.float64)
     ry = None if rec_y is None else np.ascontiguousarray(rec_y, dtype=np.float64)
-    rc = _L().oseh_write_su(path.encode(), tr.ctypes.data_as(_fp), nrec, nt, dt,
-                            None if rx is None else rx.ctypes.data_as(_dp),
-                            None if ry is None else ry.ctypes.data_as(_dp), src[0], src[1], shot)
+    rc = fn(str(path).encode(), tr.ctypes.data_as(_fp), nrec, nt, dt, None if rx is None else rx.ctypes.data_as(_dp),
+            None if ry is None else ry.ctypes.data_as(_dp), src[0], src[1], shot)
     if rc:
         raise OseError(rc, f"cannot write {path}")
 
 
-def read_su(path, cap=1 << 26):
+def _read_traces(fn, path, cap):
     buf = np.empty(cap, dtype=np.float32)
     nrec, nt, dt = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
-    rc = _L().oseh_read_su(path.encode(), buf.ctypes.data_as(_fp), cap, ctypes.byref(nrec),
-                           ctypes.byref(nt), ctypes.byref(dt))
+    rc = fn(str(path).encode(), buf.ctypes.data_as(_fp), cap, ctypes.byref(nrec), ctypes.byref(nt),
+            ctypes.byref(dt))
     if rc:
         raise OseError(rc, f"cannot read {path}")
     return buf[: nrec.value * nt.value].reshape(nrec.value, nt.value).copy(), dt.value
+
+
+def write_su(path, traces, dt, rec_x=None, rec_y=None, src=(0.0, 0.0), shot=0):
+    _write_traces(_L().oseh_write_su, path, traces, dt, rec_x, rec_y, src, shot)
+
+
+def read_su(path, cap=1 << 26):
+    return _read_traces(_L().oseh_read_su, path, cap)
+
+
+def write_segy(path, traces, dt, rec_x=None, rec_y=None, src=(0.0, 0.0), shot=0):
+    _write_traces(_L().oseh_write_segy, path, traces, dt, rec_x, rec_y, src, shot)
+
+
+def read_segy(path, cap=1 << 26):
+    return _read_traces(_L().oseh_read_segy, path, cap)
 
 
 def write_snapshot(path, f_zyx):

@@ -89,6 +89,18 @@ int oseh_write_su(const char* path, const float* traces, int nrec, int nt, doubl
                   int shot_index);
 /* Read back an SU file written above: returns nrec, nt and the samples.   */
 int oseh_read_su(const char* path, float* traces, size_t cap, int* nrec, int* nt, double* dt);
+/* SEG-Y rev 1 file: 3200-byte EBCDIC textual header, 400-byte big-endian
+ * binary header (sample interval @3216, samples per trace @3220, format 5 =
+ * IEEE float @3224, revision 0x0100 @3500), then per trace the same
+ * 240-byte header as the SU file in big-endian order and nt big-endian
+ * IEEE float32 samples.  nt and the interval in microseconds must fit in
+ * 15 bits.                                                                 */
+int oseh_write_segy(const char* path, const float* traces, int nrec, int nt, double dt,
+                    const double* rec_x, const double* rec_y, double src_x, double src_y,
+                    int shot_index);
+/* Read a SEG-Y file (format 5 IEEE or format 1 IBM floats, fixed trace
+ * length): returns nrec, nt, dt and the samples as IEEE float32.          */
+int oseh_read_segy(const char* path, float* traces, size_t cap, int* nrec, int* nt, double* dt);
 /* Wavefield snapshot over the padded grid, file order [y][x][z] with z
  * fastest (write_3dfile_from_1darray_ci ose_3d_lib.cu:364-380).            */
 int oseh_write_snapshot(const char* path, const float* f_zyx, int nxb, int nyb, int nzb);

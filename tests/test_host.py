@@ -190,3 +190,48 @@ def test_boundary_parameter_file_formats(H, tmp_path):
     from ose_amd import boundary
     w = H.read_params(boundary.WEIGHTS)
     assert w.size == 20784 and np.all(np.isfinite(w))
+
+
+def test_segy_round_trip_and_headers(H, tmp_path):
+    """SEG-Y rev 1: EBCDIC textual header, big-endian binary and trace
+    headers, big-endian IEEE samples."""
+    rng = np.random.default_rng(5)
+    tr = rng.standard_normal((5, 43)).astype(np.float32)
+    p = tmp_path / "x.segy"
+    H.write_segy(p, tr, 2e-3, rec_x=np.arange(5) * 10.0, rec_y=np.full(5, 20.0), src=(15.0, 20.0), shot=1)
+    got, dt = H.read_segy(p)
+    assert np.array_equal(got, tr) and abs(dt - 2e-3) < 1e-12
+    raw = open(p, "rb").read()
+    assert len(raw) == 3600 + 5 * (240 + 4 * 43)
+    assert raw[:3] == bytes([0xC3, 0x40, 0xF1])                       # "C 1" in EBCDIC
+    assert struct.unpack_from(">h", raw, 3216)[0] == 2000             # sample interval (us)
+    assert struct.unpack_from(">h", raw, 3220)[0] == 43               # samples per trace
+    assert struct.unpack_from(">h", raw, 3224)[0] == 5                # IEEE float
+    assert struct.unpack_from(">H", raw, 3500)[0] == 0x0100           # revision 1
+    h = raw[3600 + 2 * (240 + 4 * 43):][:240]                          # third trace
+    assert struct.unpack_from(">i", h, 0)[0] == 3                     # tracl
+    assert struct.unpack_from(">i", h, 8)[0] == 2                     # fldr = shot + 1
+    assert struct.unpack_from(">h", h, 70)[0] == -100                 # scalco
+    assert struct.unpack_from(">i", h, 72)[0] == 1500                 # sx = 15 m * 100
+    assert struct.unpack_from(">i", h, 80)[0] == 2000                 # gx = 20 m * 100
+    assert struct.unpack_from(">i", h, 36)[0] == 5                    # offset (m)
+    assert struct.unpack_from(">H", h, 114)[0] == 43
+    samples = np.frombuffer(raw[3600 + 240:3600 + 240 + 4 * 43], dtype=">f4")
+    assert np.array_equal(samples, tr[0])
+
+
+def test_segy_reads_ibm_floats(H, tmp_path):
+    """Format code 1 (IBM System/360 floats): known encodings."""
+    vals = {0x41100000: 1.0, 0xC276A000: -118.625, 0x00000000: 0.0, 0x42640000: 100.0, 0x40800000: 0.5}
+    head = bytearray(3600)
+    struct.pack_into(">h", head, 3216, 1000)
+    struct.pack_into(">h", head, 3220, len(vals))
+    struct.pack_into(">h", head, 3224, 1)
+    th = bytearray(240)
+    struct.pack_into(">H", th, 114, len(vals))
+    body = b"".join(struct.pack(">I", k) for k in vals)
+    p = tmp_path / "ibm.segy"
+    p.write_bytes(bytes(head) + bytes(th) + body)
+    got, dt = H.read_segy(p)
+    assert got.shape == (1, len(vals)) and abs(dt - 1e-3) < 1e-12
+    assert np.allclose(got[0], list(vals.values()), rtol=0, atol=1e-6)
