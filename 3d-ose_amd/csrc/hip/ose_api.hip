@@ -1179,6 +1179,31 @@ int ose_get_traces(ose_ctx* c, int shot_id, int it0, int nsteps, float* out)
     return OSE_OK;
 }
 
+int ose_set_traces(ose_ctx* c, int shot_id, int it0, int nsteps, const float* in)
+{
+    if (!c) return OSE_EINVAL;
+    Shot* sh = get_shot(c, shot_id);
+    if (!sh || !in) return c->fail(OSE_EINVAL, "ose_set_traces: bad shot or null input");
+    int rc;
+    if ((rc = set_device(c)) || (rc = finalize(c))) return rc;
+    Model& m = *c->models[sh->model];
+    if (it0 < 0 || nsteps < 0 || it0 + nsteps > m.nt_cap)
+        return c->fail(OSE_EINVAL, "ose_set_traces: steps [%d,%d) outside the trace buffer (%d)", it0,
+                       it0 + nsteps, m.nt_cap);
+    if ((rc = ose_sync(c))) return rc;
+    const long nrec = (long)m.rec_ny * m.rec_nx;
+    std::vector<float> buf((size_t)nrec * nsteps);
+    for (int it = 0; it < nsteps; ++it)
+        for (long r = 0; r < nrec; ++r) {
+            const float v = in[r * nsteps + it];
+            if (!std::isfinite(v)) return c->fail(OSE_ENUMERIC, "ose_set_traces: non-finite sample");
+            buf[(size_t)it * nrec + r] = v;
+        }
+    float* dst = m.d_traces + (long)sh->slot * m.nt_cap * nrec + (long)it0 * nrec;
+    if (nsteps) HIPCHK(c, hipMemcpy(dst, buf.data(), sizeof(float) * buf.size(), hipMemcpyHostToDevice));
+    return OSE_OK;
+}
+
 int ose_get_wavefield(ose_ctx* c, int shot_id, float* re, float* im)
 {
     if (!c) return OSE_EINVAL;

@@ -4,6 +4,7 @@
 //
 //   ose3d preset=ref [vfile=input/v3d_nx101_ny101_nz101.bin] out=output
 //   ose3d preset=c1 nt=200 out=/tmp/run snap_every=50
+//   ose3d preset=c4 checkpoint=/scratch/ck checkpoint_every=500 [restore=/scratch/ck]
 //
 // Prints one JSON line with the model set-up and the measured rate.
 #include <sys/stat.h>
@@ -18,16 +19,20 @@
 int main(int argc, char** argv)
 {
     std::string params;
-    std::string out = "output";
-    int snap_every = 0;
+    std::string out = "output", ckpt, restore;
+    int snap_every = 0, ckpt_every = 0;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
         if (a == "-h" || a == "--help") {
-            std::printf("usage: ose3d [preset=ref|c0|c1|c2|c3|c4] [key=value ...] [out=dir] [snap_every=N]\n");
+            std::printf("usage: ose3d [preset=ref|c0|c1|c2|c3|c4] [key=value ...] [out=dir] [snap_every=N]\n"
+                        "             [checkpoint=dir checkpoint_every=N] [restore=dir]\n");
             return 0;
         }
         if (a.rfind("out=", 0) == 0) out = a.substr(4);
         else if (a.rfind("snap_every=", 0) == 0) snap_every = std::atoi(a.c_str() + 11);
+        else if (a.rfind("checkpoint_every=", 0) == 0) ckpt_every = std::atoi(a.c_str() + 17);
+        else if (a.rfind("checkpoint=", 0) == 0) ckpt = a.substr(11);
+        else if (a.rfind("restore=", 0) == 0) restore = a.substr(8);
         params += a + " ";
     }
     oseh_sim* sim = nullptr;
@@ -52,8 +57,23 @@ int main(int argc, char** argv)
     const double cells = q == std::string::npos ? 0.0 : std::atof(inf.c_str() + q + 18);
     double total_ms = 0.0;
     int done = 0;
+    if (!restore.empty()) {
+        if ((rc = oseh_sim_restore(sim, restore.c_str()))) {
+            std::fprintf(stderr, "ose3d: restore failed: %s\n", oseh_sim_error(sim));
+            oseh_sim_destroy(sim);
+            return 1;
+        }
+        int step = 0;
+        ose_get_step(oseh_sim_ctx(sim), &step);
+        done = step;
+        std::fprintf(stderr, "ose3d: resumed at step %d\n", done);
+    }
+    const int first = done;
+    if (!ckpt.empty()) mkdir(ckpt.c_str(), 0755);
     while (done < nt) {
-        const int chunk = snap_every > 0 ? std::min(snap_every, nt - done) : nt - done;
+        int chunk = nt - done;
+        if (snap_every > 0) chunk = std::min(chunk, snap_every - done % snap_every);
+        if (ckpt_every > 0) chunk = std::min(chunk, ckpt_every - done % ckpt_every);
         double ms = 0.0;
         if ((rc = oseh_sim_run(sim, chunk, &ms))) {
             std::fprintf(stderr, "ose3d: run failed: %s\n", oseh_sim_error(sim));
@@ -62,8 +82,14 @@ int main(int argc, char** argv)
         }
         total_ms += ms;
         done += chunk;
-        if (snap_every > 0 && (rc = oseh_sim_snapshot(sim, out.c_str()))) {
+        if (snap_every > 0 && done % snap_every == 0 && (rc = oseh_sim_snapshot(sim, out.c_str()))) {
             std::fprintf(stderr, "ose3d: snapshot failed: %s\n", oseh_sim_error(sim));
+            oseh_sim_destroy(sim);
+            return 1;
+        }
+        if (!ckpt.empty() && ckpt_every > 0 && done % ckpt_every == 0 && done < nt &&
+            (rc = oseh_sim_checkpoint(sim, ckpt.c_str()))) {
+            std::fprintf(stderr, "ose3d: checkpoint failed: %s\n", oseh_sim_error(sim));
             oseh_sim_destroy(sim);
             return 1;
         }
@@ -74,8 +100,9 @@ int main(int argc, char** argv)
         oseh_sim_destroy(sim);
         return 1;
     }
-    std::printf("{\"setup\": %s, \"steps\": %d, \"ms\": %.3f, \"gcell_steps_per_s\": %.4f}\n", info.data(), done,
-                total_ms, total_ms > 0 ? cells * done / (total_ms * 1e6) : 0.0);
+    std::printf("{\"setup\": %s, \"steps\": %d, \"first_step\": %d, \"ms\": %.3f, \"gcell_steps_per_s\": %.4f}\n",
+                info.data(), done, first, total_ms,
+                total_ms > 0 ? cells * (done - first) / (total_ms * 1e6) : 0.0);
     oseh_sim_destroy(sim);
     return 0;
 }

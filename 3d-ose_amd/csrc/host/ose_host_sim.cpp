@@ -524,6 +524,84 @@ int oseh_sim_snapshot(oseh_sim* s, const char* dir)
     return OSE_OK;
 }
 
+// Checkpoint: ckpt.txt (step, shot count, per shot the padded extents and
+// receiver count), and per shot the wavefield (re, im: float32 [z][y][x])
+// and the traces recorded so far ([nrec][step] float32).
+int oseh_sim_checkpoint(oseh_sim* s, const char* dir)
+{
+    if (!s || !dir) return OSE_EINVAL;
+    if (!s->ctx) return s->fail(OSE_ESTATE, "not set up");
+    int rc, step = 0;
+    if ((rc = ose_get_step(s->ctx, &step))) return s->fail(rc, std::string("ose_get_step: ") + ose_last_error(s->ctx));
+    const std::string d(dir);
+    std::ostringstream meta;
+    meta << "ose-checkpoint 1\nstep " << step << "\nshots " << s->shots.size() << "\n";
+    for (size_t k = 0; k < s->shots.size(); ++k) {
+        const BuiltModel& m = s->models[s->shots[k].model];
+        const size_t n = (size_t)m.nxb * m.nyb * m.nzb;
+        const int nrec = m.spec.nx * m.spec.ny;
+        meta << "shot " << k << " " << m.nxb << " " << m.nyb << " " << m.nzb << " " << nrec << "\n";
+        std::vector<float> re(n), im(n), tr((size_t)nrec * step);
+        if ((rc = ose_get_wavefield(s->ctx, (int)k, re.data(), im.data())) ||
+            (step > 0 && (rc = ose_get_traces(s->ctx, (int)k, 0, step, tr.data()))))
+            return s->fail(rc, std::string("checkpoint read-back: ") + ose_last_error(s->ctx));
+        const std::string base = d + "/ckpt-shot" + std::to_string(k);
+        if ((rc = oseh_write_raw((base + "-re.bin").c_str(), re.data(), n)) ||
+            (rc = oseh_write_raw((base + "-im.bin").c_str(), im.data(), n)) ||
+            (rc = oseh_write_raw((base + "-traces.bin").c_str(), tr.data(), tr.size())))
+            return s->fail(rc, "cannot write checkpoint " + base);
+    }
+    std::ofstream f(d + "/ckpt.txt");
+    f << meta.str();
+    if (!f) return s->fail(OSE_EINVAL, "cannot write " + d + "/ckpt.txt");
+    return OSE_OK;
+}
+
+static int read_floats(const std::string& path, std::vector<float>& v)
+{
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return OSE_EINVAL;
+    f.read(reinterpret_cast<char*>(v.data()), (std::streamsize)(v.size() * sizeof(float)));
+    return (size_t)f.gcount() == v.size() * sizeof(float) && f.peek() == EOF ? OSE_OK : OSE_EINVAL;
+}
+
+int oseh_sim_restore(oseh_sim* s, const char* dir)
+{
+    if (!s || !dir) return OSE_EINVAL;
+    if (!s->ctx) return s->fail(OSE_ESTATE, "not set up");
+    const std::string d(dir);
+    std::ifstream f(d + "/ckpt.txt");
+    std::string tag, key;
+    int version = 0, step = -1;
+    size_t nshots = 0;
+    if (!(f >> tag >> version) || tag != "ose-checkpoint" || version != 1 || !(f >> key >> step) || key != "step" ||
+        !(f >> key >> nshots) || key != "shots")
+        return s->fail(OSE_EINVAL, "not a checkpoint: " + d + "/ckpt.txt");
+    if (nshots != s->shots.size() || step < 0 || step > s->geti("nt", 300))
+        return s->fail(OSE_EINVAL, "checkpoint does not match this configuration");
+    int rc;
+    for (size_t k = 0; k < nshots; ++k) {
+        const BuiltModel& m = s->models[s->shots[k].model];
+        size_t idx = 0;
+        int nxb = 0, nyb = 0, nzb = 0, nrec = 0;
+        if (!(f >> key >> idx >> nxb >> nyb >> nzb >> nrec) || key != "shot" || idx != k || nxb != m.nxb ||
+            nyb != m.nyb || nzb != m.nzb || nrec != m.spec.nx * m.spec.ny)
+            return s->fail(OSE_EINVAL, "checkpoint shot " + std::to_string(k) + " does not match this configuration");
+        const size_t n = (size_t)nxb * nyb * nzb;
+        std::vector<float> re(n), im(n), tr((size_t)nrec * step);
+        const std::string base = d + "/ckpt-shot" + std::to_string(k);
+        if (read_floats(base + "-re.bin", re) || read_floats(base + "-im.bin", im) ||
+            read_floats(base + "-traces.bin", tr))
+            return s->fail(OSE_EINVAL, "cannot read checkpoint " + base);
+        if ((rc = ose_set_wavefield(s->ctx, (int)k, re.data(), im.data())) ||
+            (step > 0 && (rc = ose_set_traces(s->ctx, (int)k, 0, step, tr.data()))))
+            return s->fail(rc, std::string("checkpoint upload: ") + ose_last_error(s->ctx));
+    }
+    if ((rc = ose_set_step(s->ctx, step))) return s->fail(rc, std::string("ose_set_step: ") + ose_last_error(s->ctx));
+    s->steps_done = step;
+    return OSE_OK;
+}
+
 int oseh_sim_nmodels(oseh_sim* s) { return s ? (int)s->models.size() : 0; }
 
 int oseh_sim_model_dims(oseh_sim* s, int mi, int* d)

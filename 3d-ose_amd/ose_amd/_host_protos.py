@@ -38,6 +38,8 @@ PROTOS = {
     "oseh_sim_info": ([vp, ctypes.c_char_p, c_size_t], c_int),
     "oseh_sim_write": ([vp, cp], c_int),
     "oseh_sim_snapshot": ([vp, cp], c_int),
+    "oseh_sim_checkpoint": ([vp, cp], c_int),
+    "oseh_sim_restore": ([vp, cp], c_int),
     "oseh_sim_error": ([vp], cp),
     "oseh_sim_destroy": ([vp], None),
     "oseh_sim_nmodels": ([vp], c_int),

@@ -210,6 +210,12 @@ class Simulation:
     def write(self, outdir):
         self._check(self._L.oseh_sim_write(self._h, outdir.encode()))
 
+    def checkpoint(self, d):
+        self._check(self._L.oseh_sim_checkpoint(self._h, str(d).encode()))
+
+    def restore(self, d):
+        self._check(self._L.oseh_sim_restore(self._h, str(d).encode()))
+
     @property
     def ctx(self):
         return ctypes.c_void_p(self._L.oseh_sim_ctx(self._h))

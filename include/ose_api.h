@@ -90,6 +90,9 @@ int ose_timed_run(ose_ctx* ctx, int nsteps, double* ms);
  * reference's seismogram (rec_shotdata_3d ose_3d_kernel.cu:412-436,
  * rec_slice_to_data ose_3d_lib.cu:876-884, write ose_3d.cu:276).          */
 int ose_get_traces(ose_ctx* ctx, int shot_id, int it0, int nsteps, float* out);
+/* Upload recorded traces (same layout as ose_get_traces), e.g. when a run
+ * resumes from a checkpoint.  No counterpart in the reference.            */
+int ose_set_traces(ose_ctx* ctx, int shot_id, int it0, int nsteps, const float* in);
 
 /* Current state u = P + iQ of a shot over the padded grid (the p2D/q2D .y
  * lanes; snapshot dump ose_3d.cu:260-270).                                 */

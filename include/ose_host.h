@@ -138,6 +138,12 @@ int  oseh_sim_write(oseh_sim* sim, const char* dir);
 /* Write the current wavefield of every shot as p-<it>-shot<k>.bin (Re u)
  * and q-<it>-shot<k>.bin (Im u) in the snapshot order (ose_3d.cu:260-270). */
 int  oseh_sim_snapshot(oseh_sim* sim, const char* dir);
+/* Checkpoint / resume of a set-up simulation: the step counter, every
+ * shot's wavefield and the traces recorded so far go to dir; restore reads
+ * them into a simulation built with the same parameters and continues from
+ * that step.  Long multi-shot runs restart without recomputing.           */
+int  oseh_sim_checkpoint(oseh_sim* sim, const char* dir);
+int  oseh_sim_restore(oseh_sim* sim, const char* dir);
 const char* oseh_sim_error(oseh_sim* sim);
 void oseh_sim_destroy(oseh_sim* sim);
 /* Model / shot data of a simulation, for checking against the oracle.    */
