@@ -58,6 +58,39 @@ __global__ void k_advance(int* it)
     if (threadIdx.x == 0 && blockIdx.x == 0) *it = *it + 1;
 }
 
+// One receiver of a listed geometry: its padded cell, its shot's slot in the
+// model's batch and where its samples go (trace buffer offset base + it *
+// stride; the samples of one step of one shot are contiguous).
+struct RecvDesc {
+    long cell;
+    long base;
+    int slot;
+    int stride;
+};
+
+struct RecvArgs {
+    const float2* u;        // W_0 of the model's batch: the new state
+    long shot_stride;
+    const RecvDesc* d;
+    int n;
+    int nt_cap;
+    const int* it;
+    float* tr;
+};
+
+// Receivers at listed cells (the geometry of sg_init_device,
+// ose_3d_kernel.cu:202-212, read back as in the cell lookup of
+// rec_shotdata_3d, ose_3d_kernel.cu:417-423): Im(u) after the step.
+__global__ void k_gather_recv(RecvArgs a)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= a.n) return;
+    const int itv = *a.it;
+    if (itv >= a.nt_cap) return;
+    const RecvDesc e = a.d[r];
+    a.tr[e.base + (long)itv * e.stride] = a.u[(long)e.slot * a.shot_stride + e.cell].y;
+}
+
 // plain streaming copy, the bandwidth yardstick of the passes: each thread
 // moves four independent 16-byte vectors (64 B), one pass over the array
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -164,7 +197,12 @@ struct Model {
     long* d_src_line = nullptr;
     int* d_src_x = nullptr;
     float *d_s = nullptr, *d_h = nullptr;
-    int rec_ny = 0, rec_nx = 0;
+    int rec_ny = 0, rec_nx = 0;    // plane receivers (0 x 0 when every shot has a list)
+    // listed receivers of the batch (ose_set_receivers)
+    DevBuf rdesc, rtr;
+    RecvDesc* d_rdesc = nullptr;
+    float* d_rtr = nullptr;
+    int nrecv = 0;
     // learned thin pad (boundary network)
     bool learned = false;          // a network is set for this model
     bool learned_on = false;       // used by the step (cleared once the guard trips)
@@ -185,6 +223,8 @@ struct Shot {
     int sx = 0, sy = 0, sz = 0;
     std::vector<float> s, h;
     int nt = 0;
+    std::vector<long> rec;         // listed receivers (padded cells); empty: the plane
+    long rbase = 0;                // offset of this shot's samples in the model's list traces
 };
 
 struct Pending {
@@ -505,6 +545,18 @@ int enqueue_model_step(ose_ctx* c, Model& m, int* d_it)
         if ((rc = launch_x(c, -1, m.ex->xk[m.sel_x], XM_FWD, a, ny * nz, nshot, 16.0 * cells)))
             return rc;
     }
+    if (m.nrecv > 0) {
+        RecvArgs r{};
+        r.u = m.W[0];
+        r.shot_stride = m.ncell;
+        r.d = m.d_rdesc;
+        r.n = m.nrecv;
+        r.nt_cap = m.nt_cap;
+        r.it = d_it;
+        r.tr = m.d_rtr;
+        if ((rc = launch(c, -1, 0.0, (const void*)&k_gather_recv, dim3((m.nrecv + 255) / 256), dim3(256), 0, &r)))
+            return rc;
+    }
     return OSE_OK;
 }
 
@@ -625,8 +677,34 @@ int finalize(ose_ctx* c)
         if (nshot == 0) continue;
         m.nt_cap = 0;
         for (int id : m.shots) m.nt_cap = std::max(m.nt_cap, c->shots[id].nt);
-        m.rec_ny = m.d.ny;
-        m.rec_nx = m.d.nx;
+        // the plane gather runs unless every shot of the model lists its receivers
+        bool plane = false;
+        long rsamples = 0;
+        m.nrecv = 0;
+        for (int id : m.shots) {
+            Shot& sh = c->shots[id];
+            plane |= sh.rec.empty();
+            sh.rbase = rsamples;
+            rsamples += (long)m.nt_cap * (long)sh.rec.size();
+            m.nrecv += (int)sh.rec.size();
+        }
+        m.rec_ny = plane ? m.d.ny : 0;
+        m.rec_nx = plane ? m.d.nx : 0;
+        if (m.nrecv > 0) {
+            std::vector<RecvDesc> desc;
+            desc.reserve(m.nrecv);
+            for (int id : m.shots) {
+                const Shot& sh = c->shots[id];
+                for (size_t r = 0; r < sh.rec.size(); ++r)
+                    desc.push_back({sh.rec[r], sh.rbase + (long)r, sh.slot, (int)sh.rec.size()});
+            }
+            if ((rc = dev_alloc(c, m.rdesc, sizeof(RecvDesc) * desc.size()))) return rc;
+            m.d_rdesc = (RecvDesc*)m.rdesc.p;
+            HIPCHK(c, hipMemcpy(m.d_rdesc, desc.data(), sizeof(RecvDesc) * desc.size(), hipMemcpyHostToDevice));
+            if ((rc = dev_alloc(c, m.rtr, sizeof(float) * (size_t)rsamples))) return rc;
+            m.d_rtr = (float*)m.rtr.p;
+            HIPCHK(c, hipMemsetAsync(m.rtr.p, 0, sizeof(float) * (size_t)rsamples, c->stream));
+        }
         const size_t fbytes = sizeof(float2) * (size_t)m.ncell * nshot;
         if ((rc = dev_alloc(c, m.fields, 4 * fbytes))) return rc;
         char* base = (char*)m.fields.p;
@@ -882,6 +960,35 @@ int ose_add_shot(ose_ctx* c, int model_id, int sx, int sy, int sz, const float* 
     c->shots.push_back(sh);
     m.shots.push_back((int)c->shots.size() - 1);
     if (shot_id) *shot_id = (int)c->shots.size() - 1;
+    return OSE_OK;
+}
+
+int ose_set_receivers(ose_ctx* c, int shot_id, const int* xyz, int n)
+{
+    if (!c) return OSE_EINVAL;
+    if (c->finalized) return c->fail(OSE_ESTATE, "ose_set_receivers: context already running");
+    Shot* sh = get_shot(c, shot_id);
+    if (!sh || n < 0 || (n > 0 && !xyz)) return c->fail(OSE_EINVAL, "ose_set_receivers: bad shot or receiver list");
+    const Model& m = *c->models[sh->model];
+    std::vector<long> rec((size_t)n);
+    for (int r = 0; r < n; ++r) {
+        const int x = xyz[3 * r], y = xyz[3 * r + 1], z = xyz[3 * r + 2];
+        if (x < 0 || x >= m.d.nxb || y < 0 || y >= m.d.nyb || z < 0 || z >= m.d.nzb)
+            return c->fail(OSE_EINVAL, "ose_set_receivers: receiver %d at (%d, %d, %d) is outside the %dx%dx%d grid", r,
+                           x, y, z, m.d.nxb, m.d.nyb, m.d.nzb);
+        rec[(size_t)r] = ((long)z * m.d.nyb + y) * m.d.nxb + x;
+    }
+    sh->rec.swap(rec);
+    return OSE_OK;
+}
+
+int ose_receiver_count(ose_ctx* c, int shot_id, int* nrec)
+{
+    if (!c) return OSE_EINVAL;
+    const Shot* sh = get_shot(c, shot_id);
+    if (!sh || !nrec) return c->fail(OSE_EINVAL, "ose_receiver_count: bad shot or null output");
+    const Model& m = *c->models[sh->model];
+    *nrec = sh->rec.empty() ? m.d.nx * m.d.ny : (int)sh->rec.size();
     return OSE_OK;
 }
 
@@ -1170,9 +1277,10 @@ int ose_get_traces(ose_ctx* c, int shot_id, int it0, int nsteps, float* out)
         return c->fail(OSE_EINVAL, "ose_get_traces: steps [%d,%d) outside the trace buffer (%d)", it0,
                        it0 + nsteps, m.nt_cap);
     if ((rc = ose_sync(c))) return rc;
-    const long nrec = (long)m.rec_ny * m.rec_nx;
+    const long nrec = sh->rec.empty() ? (long)m.rec_ny * m.rec_nx : (long)sh->rec.size();
     std::vector<float> buf((size_t)nrec * nsteps);
-    const float* src = m.d_traces + (long)sh->slot * m.nt_cap * nrec + (long)it0 * nrec;
+    const float* src = sh->rec.empty() ? m.d_traces + (long)sh->slot * m.nt_cap * nrec + (long)it0 * nrec
+                                       : m.d_rtr + sh->rbase + (long)it0 * nrec;
     if (nsteps) HIPCHK(c, hipMemcpy(buf.data(), src, sizeof(float) * buf.size(), hipMemcpyDeviceToHost));
     for (int it = 0; it < nsteps; ++it)
         for (long r = 0; r < nrec; ++r) out[r * nsteps + it] = buf[(size_t)it * nrec + r];
@@ -1191,7 +1299,7 @@ int ose_set_traces(ose_ctx* c, int shot_id, int it0, int nsteps, const float* in
         return c->fail(OSE_EINVAL, "ose_set_traces: steps [%d,%d) outside the trace buffer (%d)", it0,
                        it0 + nsteps, m.nt_cap);
     if ((rc = ose_sync(c))) return rc;
-    const long nrec = (long)m.rec_ny * m.rec_nx;
+    const long nrec = sh->rec.empty() ? (long)m.rec_ny * m.rec_nx : (long)sh->rec.size();
     std::vector<float> buf((size_t)nrec * nsteps);
     for (int it = 0; it < nsteps; ++it)
         for (long r = 0; r < nrec; ++r) {
@@ -1199,7 +1307,8 @@ int ose_set_traces(ose_ctx* c, int shot_id, int it0, int nsteps, const float* in
             if (!std::isfinite(v)) return c->fail(OSE_ENUMERIC, "ose_set_traces: non-finite sample");
             buf[(size_t)it * nrec + r] = v;
         }
-    float* dst = m.d_traces + (long)sh->slot * m.nt_cap * nrec + (long)it0 * nrec;
+    float* dst = sh->rec.empty() ? m.d_traces + (long)sh->slot * m.nt_cap * nrec + (long)it0 * nrec
+                                 : m.d_rtr + sh->rbase + (long)it0 * nrec;
     if (nsteps) HIPCHK(c, hipMemcpy(dst, buf.data(), sizeof(float) * buf.size(), hipMemcpyHostToDevice));
     return OSE_OK;
 }

@@ -45,8 +45,8 @@ uint16_t get_be16(const unsigned char* h, int off) { return (uint16_t)((h[off] <
 // The 240-byte trace header shared by SU (native byte order) and SEG-Y
 // (big-endian); byte offsets of the SEG-Y rev 1 standard.
 template <bool BE>
-void trace_header(unsigned char* h, int r, int nt, double us, double gx, double gy, double src_x, double src_y,
-                  int shot_index)
+void trace_header(unsigned char* h, int r, int nt, double us, double gx, double gy, double gz, double src_x,
+                  double src_y, double src_z, int shot_index)
 {
     std::memset(h, 0, 240);
     auto i32 = [h](int off, int32_t v) {
@@ -65,6 +65,9 @@ void trace_header(unsigned char* h, int r, int nt, double us, double gx, double 
     i32(20, r + 1);                                // cdp
     i16(28, 1);                                    // trid: seismic data
     i32(36, (int32_t)std::lround(std::hypot(gx - src_x, gy - src_y)));   // offset
+    i32(40, (int32_t)std::lround(-gz * 100.0));    // gelev: receiver elevation (depth down)
+    i32(48, (int32_t)std::lround(src_z * 100.0));  // sdepth: source depth
+    i16(68, (int16_t)-100);                        // scalel: elevations / 100
     i16(70, (int16_t)-100);                        // scalco: coordinates / 100
     i32(72, (int32_t)std::lround(src_x * 100.0));
     i32(76, (int32_t)std::lround(src_y * 100.0));
@@ -122,7 +125,8 @@ int oseh_write_raw(const char* path, const float* data, size_t n)
 }
 
 int oseh_write_su(const char* path, const float* traces, int nrec, int nt, double dt,
-                  const double* rec_x, const double* rec_y, double src_x, double src_y,
+                  const double* rec_x, const double* rec_y, const double* rec_z, double src_x, double src_y,
+                  double src_z,
                   int shot_index)
 {
     if (!path || !traces || nrec < 0 || nt < 0 || nt > 65535) return OSE_EINVAL;
@@ -133,8 +137,8 @@ int oseh_write_su(const char* path, const float* traces, int nrec, int nt, doubl
     unsigned char h[240];
     bool ok = true;
     for (int r = 0; r < nrec && ok; ++r) {
-        const double gx = rec_x ? rec_x[r] : 0.0, gy = rec_y ? rec_y[r] : 0.0;
-        trace_header<false>(h, r, nt, us, gx, gy, src_x, src_y, shot_index);
+        const double gx = rec_x ? rec_x[r] : 0.0, gy = rec_y ? rec_y[r] : 0.0, gz = rec_z ? rec_z[r] : 0.0;
+        trace_header<false>(h, r, nt, us, gx, gy, gz, src_x, src_y, src_z, shot_index);
         ok = std::fwrite(h, 1, sizeof h, f) == sizeof h &&
              std::fwrite(traces + (size_t)r * nt, sizeof(float), (size_t)nt, f) == (size_t)nt;
     }
@@ -182,7 +186,8 @@ int oseh_read_su(const char* path, float* traces, size_t cap, int* nrec, int* nt
 }
 
 int oseh_write_segy(const char* path, const float* traces, int nrec, int nt, double dt, const double* rec_x,
-                    const double* rec_y, double src_x, double src_y, int shot_index)
+                    const double* rec_y, const double* rec_z, double src_x, double src_y, double src_z,
+                    int shot_index)
 {
     if (!path || !traces || nrec < 0 || nt < 0 || nt > 32767) return OSE_EINVAL;
     const double us = dt * 1e6;
@@ -225,8 +230,8 @@ int oseh_write_segy(const char* path, const float* traces, int nrec, int nt, dou
     unsigned char h[240];
     std::vector<unsigned char> data((size_t)nt * 4);
     for (int r = 0; r < nrec && ok; ++r) {
-        const double gx = rec_x ? rec_x[r] : 0.0, gy = rec_y ? rec_y[r] : 0.0;
-        trace_header<true>(h, r, nt, us, gx, gy, src_x, src_y, shot_index);
+        const double gx = rec_x ? rec_x[r] : 0.0, gy = rec_y ? rec_y[r] : 0.0, gz = rec_z ? rec_z[r] : 0.0;
+        trace_header<true>(h, r, nt, us, gx, gy, gz, src_x, src_y, src_z, shot_index);
         for (int i = 0; i < nt; ++i) {
             uint32_t u;
             std::memcpy(&u, traces + (size_t)r * nt + i, 4);

@@ -43,7 +43,8 @@ struct BuiltModel {
 
 struct ShotRec {
     int model;
-    int src[3];   // padded
+    int src[3];              // padded
+    std::vector<int> rec;    // listed receivers, interior (x, y, z) triples; empty: the plane rec_z
 };
 
 }  // namespace
@@ -316,6 +317,59 @@ int oseh_sim_create(oseh_sim** out, const char* params)
 
 const char* oseh_sim_error(oseh_sim* s) { return s ? s->err.c_str() : "null simulation"; }
 
+// Receiver geometry of every shot.  Without ng/ngx/ngy the shot records the
+// whole interior plane rec_z (what rec_shotdata_3d does, ose_3d_kernel.cu:
+// 425-436).  ng=<n>: a line of n receivers at (gxbeg + r*jgx, gybeg + r*jgy,
+// gzbeg + r*jgz), the geometry sg_init_device builds from the reference's
+// gxbeg..jgz (ose_3d.cu:105-114, ose_3d_kernel.cu:202-212); defaults
+// jgx = 1, jgy = jgz = 0.  ngx/ngy=<n>: an ngx x ngy grid at depth gzbeg,
+// x fastest, spacing jgx, jgy (default 1).  csdgather=1 moves the spread with
+// the source: shot k's receivers shift by its source's offset from the
+// model's first source (a common-shot gather with a fixed source-receiver
+// layout).  Coordinates are interior cells; a receiver outside the interior
+// is an error.
+static int build_receivers(oseh_sim* s)
+{
+    const int ng = s->geti("ng", 0), ngx = s->geti("ngx", 0), ngy = s->geti("ngy", 0);
+    if (ng < 0 || ngx < 0 || ngy < 0) return s->fail(OSE_EINVAL, "ng, ngx and ngy must not be negative");
+    const bool grid = ngx > 0 || ngy > 0;
+    if (ng == 0 && !grid) return OSE_OK;
+    if (ng > 0 && grid) return s->fail(OSE_EINVAL, "give either ng (a receiver line) or ngx/ngy (a grid)");
+    const int gx0 = s->geti("gxbeg", 0), gy0 = s->geti("gybeg", 0), gz0 = s->geti("gzbeg", 0);
+    const int jgx = s->geti("jgx", 1), jgy = s->geti("jgy", grid ? 1 : 0), jgz = s->geti("jgz", 0);
+    const bool csd = s->geti("csdgather", 0) != 0;
+    std::vector<int> first(s->models.size(), -1);
+    for (size_t k = 0; k < s->shots.size(); ++k) {
+        ShotRec& sh = s->shots[k];
+        const BuiltModel& m = s->models[sh.model];
+        if (first[sh.model] < 0) first[sh.model] = (int)k;
+        const ShotRec& f = s->shots[first[sh.model]];
+        const int ox = csd ? sh.src[0] - f.src[0] : 0, oy = csd ? sh.src[1] - f.src[1] : 0,
+                  oz = csd ? sh.src[2] - f.src[2] : 0;
+        sh.rec.clear();
+        auto add = [&](int x, int y, int z) {
+            x += ox, y += oy, z += oz;
+            if (x < 0 || x >= m.spec.nx || y < 0 || y >= m.spec.ny || z < 0 || z >= m.spec.nz)
+                return s->fail(OSE_EINVAL, "receiver " + std::to_string(sh.rec.size() / 3) + " of shot " +
+                                               std::to_string(k) + " at (" + std::to_string(x) + ", " +
+                                               std::to_string(y) + ", " + std::to_string(z) +
+                                               ") is outside the interior");
+            sh.rec.insert(sh.rec.end(), {x, y, z});
+            return (int)OSE_OK;
+        };
+        int rc;
+        if (grid) {
+            for (int iy = 0; iy < std::max(ngy, 1); ++iy)
+                for (int ix = 0; ix < std::max(ngx, 1); ++ix)
+                    if ((rc = add(gx0 + ix * jgx, gy0 + iy * jgy, gz0))) return rc;
+        } else {
+            for (int r = 0; r < ng; ++r)
+                if ((rc = add(gx0 + r * jgx, gy0 + r * jgy, gz0 + r * jgz))) return rc;
+        }
+    }
+    return OSE_OK;
+}
+
 int oseh_sim_build(oseh_sim* s)
 {
     if (!s) return OSE_EINVAL;
@@ -343,10 +397,10 @@ int oseh_sim_build(oseh_sim* s)
         for (auto& sh : m.spec.shots) {
             if (sh.x < 0 || sh.x >= m.spec.nx || sh.y < 0 || sh.y >= m.spec.ny || sh.z < 0 || sh.z >= m.spec.nz)
                 return s->fail(OSE_EINVAL, "source outside the interior");
-            s->shots.push_back(ShotRec{(int)mi, {m.bl + sh.x, m.bf + sh.y, m.bu + sh.z}});
+            s->shots.push_back(ShotRec{(int)mi, {m.bl + sh.x, m.bf + sh.y, m.bu + sh.z}, {}});
         }
     }
-    return OSE_OK;
+    return build_receivers(s);
 }
 
 int oseh_sim_setup(oseh_sim* s)
@@ -392,6 +446,16 @@ int oseh_sim_setup(oseh_sim* s)
         if ((rc = ose_add_shot(s->ctx, m.id, r.src[0], r.src[1], r.src[2], s->s.data(), s->h.data(), nt, &sid)))
             return s->fail(rc, std::string("ose_add_shot: ") + ose_last_error(s->ctx));
         m.shot_ids.push_back(sid);
+        if (!r.rec.empty()) {
+            std::vector<int> xyz(r.rec.size());
+            for (size_t i = 0; i < r.rec.size(); i += 3) {
+                xyz[i] = r.rec[i] + m.bl;
+                xyz[i + 1] = r.rec[i + 1] + m.bf;
+                xyz[i + 2] = r.rec[i + 2] + m.bu;
+            }
+            if ((rc = ose_set_receivers(s->ctx, sid, xyz.data(), (int)(xyz.size() / 3))))
+                return s->fail(rc, std::string("ose_set_receivers: ") + ose_last_error(s->ctx));
+        }
     }
     // learned thin pad: bnet=<parameters file> [guard=4] [bnet_taper=replace|keep]
     const std::string bn = s->get("bnet", "");
@@ -463,34 +527,64 @@ int oseh_sim_info(oseh_sim* s, char* buf, size_t cap)
     return OSE_OK;
 }
 
+// interior (x, y, z) of every receiver of shot k: its list, or the plane
+// rec_z with x fastest
+static std::vector<int> receiver_cells(const oseh_sim* s, size_t k)
+{
+    const ShotRec& sh = s->shots[k];
+    if (!sh.rec.empty()) return sh.rec;
+    const ModelSpec& m = s->models[sh.model].spec;
+    const int rz = s->geti("rec_z", 0);
+    std::vector<int> xyz;
+    xyz.reserve((size_t)3 * m.nx * m.ny);
+    for (int iy = 0; iy < m.ny; ++iy)
+        for (int ix = 0; ix < m.nx; ++ix) xyz.insert(xyz.end(), {ix, iy, rz});
+    return xyz;
+}
+
+int oseh_sim_receivers(oseh_sim* s, int shot, int* xyz, int* n)
+{
+    if (!s || !n) return OSE_EINVAL;
+    if (s->models.empty()) return s->fail(OSE_ESTATE, "not built");
+    if (shot < 0 || shot >= (int)s->shots.size()) return s->fail(OSE_EINVAL, "bad shot index");
+    const std::vector<int> c = receiver_cells(s, (size_t)shot);
+    const int cnt = (int)(c.size() / 3);
+    if (xyz && *n >= cnt) std::copy(c.begin(), c.end(), xyz);
+    const int cap = *n;
+    *n = cnt;
+    return (xyz && cap < cnt) ? OSE_EINVAL : OSE_OK;
+}
+
 int oseh_sim_write(oseh_sim* s, const char* dir)
 {
     if (!s || !dir) return OSE_EINVAL;
     if (!s->ctx) return s->fail(OSE_ESTATE, "not set up");
     const int nt = std::min(s->steps_done, s->geti("nt", 300));
-    const double dx = s->getd("dx", 10), dy = s->getd("dy", 10), dt = s->getd("dt", 1e-3);
+    const double dx = s->getd("dx", 10), dy = s->getd("dy", 10), dz = s->getd("dz", 10), dt = s->getd("dt", 1e-3);
     int rc;
     for (size_t k = 0; k < s->shots.size(); ++k) {
         const BuiltModel& m = s->models[s->shots[k].model];
-        const int nrec = m.spec.nx * m.spec.ny;
+        const std::vector<int> cells = receiver_cells(s, k);
+        const int nrec = (int)(cells.size() / 3);
         std::vector<float> tr((size_t)nrec * nt);
         if ((rc = ose_get_traces(s->ctx, (int)k, 0, nt, tr.data())))
             return s->fail(rc, std::string("ose_get_traces: ") + ose_last_error(s->ctx));
         const std::string base = std::string(dir) + "/shot" + std::to_string(k);
         if ((rc = oseh_write_raw((base + ".bin").c_str(), tr.data(), tr.size())))
             return s->fail(rc, "cannot write " + base + ".bin");
-        std::vector<double> rx(nrec), ry(nrec);
-        for (int iy = 0; iy < m.spec.ny; ++iy)
-            for (int ix = 0; ix < m.spec.nx; ++ix) {
-                rx[(size_t)iy * m.spec.nx + ix] = ix * dx;
-                ry[(size_t)iy * m.spec.nx + ix] = iy * dy;
-            }
-        const double sx = (s->shots[k].src[0] - m.bl) * dx, sy = (s->shots[k].src[1] - m.bf) * dy;
+        std::vector<double> rx(nrec), ry(nrec), rz(nrec);
+        for (int r = 0; r < nrec; ++r) {
+            rx[r] = cells[3 * r] * dx;
+            ry[r] = cells[3 * r + 1] * dy;
+            rz[r] = cells[3 * r + 2] * dz;
+        }
+        const double sx = (s->shots[k].src[0] - m.bl) * dx, sy = (s->shots[k].src[1] - m.bf) * dy,
+                     sz = (s->shots[k].src[2] - m.bu) * dz;
         if (nt <= 65535 && (rc = oseh_write_su((base + ".su").c_str(), tr.data(), nrec, nt, dt, rx.data(),
-                                               ry.data(), sx, sy, (int)k)))
+                                               ry.data(), rz.data(), sx, sy, sz, (int)k)))
             return s->fail(rc, "cannot write " + base + ".su");
         if (s->geti("segy", 0) && (rc = oseh_write_segy((base + ".segy").c_str(), tr.data(), nrec, nt, dt, rx.data(),
-                                                         ry.data(), sx, sy, (int)k)))
+                                                         ry.data(), rz.data(), sx, sy, sz, (int)k)))
             return s->fail(rc, "cannot write " + base + ".segy");
     }
     if (s->geti("snap", 0) && (rc = oseh_sim_snapshot(s, dir))) return rc;
@@ -539,7 +633,7 @@ int oseh_sim_checkpoint(oseh_sim* s, const char* dir)
     for (size_t k = 0; k < s->shots.size(); ++k) {
         const BuiltModel& m = s->models[s->shots[k].model];
         const size_t n = (size_t)m.nxb * m.nyb * m.nzb;
-        const int nrec = m.spec.nx * m.spec.ny;
+        const int nrec = (int)(receiver_cells(s, k).size() / 3);
         meta << "shot " << k << " " << m.nxb << " " << m.nyb << " " << m.nzb << " " << nrec << "\n";
         std::vector<float> re(n), im(n), tr((size_t)nrec * step);
         if ((rc = ose_get_wavefield(s->ctx, (int)k, re.data(), im.data())) ||
@@ -585,7 +679,7 @@ int oseh_sim_restore(oseh_sim* s, const char* dir)
         size_t idx = 0;
         int nxb = 0, nyb = 0, nzb = 0, nrec = 0;
         if (!(f >> key >> idx >> nxb >> nyb >> nzb >> nrec) || key != "shot" || idx != k || nxb != m.nxb ||
-            nyb != m.nyb || nzb != m.nzb || nrec != m.spec.nx * m.spec.ny)
+            nyb != m.nyb || nzb != m.nzb || (size_t)nrec * 3 != receiver_cells(s, k).size())
             return s->fail(OSE_EINVAL, "checkpoint shot " + std::to_string(k) + " does not match this configuration");
         const size_t n = (size_t)nxb * nyb * nzb;
         std::vector<float> re(n), im(n), tr((size_t)nrec * step);

@@ -24,9 +24,9 @@ PROTOS = {
     "oseh_rng_normal": ([P(c_u64)], c_double),
     "oseh_rng_seed": ([P(c_u64), c_u64], None),
     "oseh_write_raw": ([cp, fp, c_size_t], c_int),
-    "oseh_write_su": ([cp, fp, c_int, c_int, c_double, dp, dp, c_double, c_double, c_int], c_int),
+    "oseh_write_su": ([cp, fp, c_int, c_int, c_double, dp, dp, dp, c_double, c_double, c_double, c_int], c_int),
     "oseh_read_su": ([cp, fp, c_size_t, ip, ip, dp], c_int),
-    "oseh_write_segy": ([cp, fp, c_int, c_int, c_double, dp, dp, c_double, c_double, c_int], c_int),
+    "oseh_write_segy": ([cp, fp, c_int, c_int, c_double, dp, dp, dp, c_double, c_double, c_double, c_int], c_int),
     "oseh_read_segy": ([cp, fp, c_size_t, ip, ip, dp], c_int),
     "oseh_write_snapshot": ([cp, fp, c_int, c_int, c_int], c_int),
     "oseh_sim_create": ([P(vp), cp], c_int),
@@ -47,6 +47,7 @@ PROTOS = {
     "oseh_sim_model_velocity": ([vp, c_int, fp], c_int),
     "oseh_sim_model_params": ([vp, c_int, dp, ip, dp, fp, fp, fp, ip], c_int),
     "oseh_sim_nshots": ([vp], c_int),
+    "oseh_sim_receivers": ([vp, c_int, ip, ip], c_int),
     "oseh_sim_shot": ([vp, c_int, ip, ip, ip, fp, fp], c_int),
 }
 

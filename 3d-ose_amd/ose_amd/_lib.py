@@ -63,6 +63,8 @@ def hip():
         _proto(L, "ose_timed_run", [vp, c_int, dp])
         _proto(L, "ose_get_traces", [vp, c_int, c_int, c_int, fp])
         _proto(L, "ose_set_traces", [vp, c_int, c_int, c_int, fp])
+        _proto(L, "ose_set_receivers", [vp, c_int, ip, c_int])
+        _proto(L, "ose_receiver_count", [vp, c_int, ip])
         _proto(L, "ose_get_wavefield", [vp, c_int, fp, fp])
         _proto(L, "ose_set_wavefield", [vp, c_int, fp, fp])
         _proto(L, "ose_get_step", [vp, ip])
@@ -113,7 +115,8 @@ def hip_symbols():
     """Names declared in include/ose_api.h (checked against the .so exports)."""
     return [
         "ose_create", "ose_destroy", "ose_last_error", "ose_add_model", "ose_add_shot",
-        "ose_run", "ose_sync", "ose_timed_run", "ose_get_traces", "ose_set_traces", "ose_get_wavefield",
+        "ose_run", "ose_sync", "ose_timed_run", "ose_get_traces", "ose_set_traces", "ose_set_receivers",
+        "ose_receiver_count", "ose_get_wavefield",
         "ose_set_wavefield", "ose_get_step", "ose_set_step", "ose_apply_operator",
         "ose_fft3d", "ose_set_options", "ose_pass_stats", "ose_reset_stats",
         "ose_supported_extent", "ose_extent_is_supported", "ose_set_autotune", "ose_get_tuning",

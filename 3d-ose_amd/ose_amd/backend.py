@@ -88,6 +88,17 @@ class Context:
         self.shots.append(dict(model=model, nt=s.size))
         return sid.value
 
+    def set_receivers(self, shot, xyz):
+        """Record the shot at listed padded cells (an (n, 3) array of x, y, z)
+        instead of the plane rec_z; an empty list restores the plane."""
+        a = np.ascontiguousarray(np.asarray(xyz, dtype=np.int32).reshape(-1, 3))
+        self._check(self._L.ose_set_receivers(self._h, shot, a.ctypes.data_as(ctypes.POINTER(c_int)), len(a)))
+
+    def receiver_count(self, shot):
+        n = c_int()
+        self._check(self._L.ose_receiver_count(self._h, shot, ctypes.byref(n)))
+        return n.value
+
     def copy_probe(self, nbytes=1 << 30, reps=10):
         """GB/s of a plain device copy (read + write bytes)."""
         g = c_double()
@@ -156,11 +167,9 @@ class Context:
 
     # -- results ------------------------------------------------------
     def traces(self, shot, it0=0, nsteps=None):
-        m = self.models[self.shots[shot]["model"]]
-        nx, ny, _ = m["interior"]
         if nsteps is None:
             nsteps = self.shots[shot]["nt"] - it0
-        out = np.empty((nx * ny, nsteps), dtype=np.float32)
+        out = np.empty((self.receiver_count(shot), nsteps), dtype=np.float32)
         self._check(self._L.ose_get_traces(self._h, shot, it0, nsteps, _ptr(out)))
         return out
 

@@ -101,13 +101,13 @@ def model(kind, nx, ny, nz, seed=1, v0=2000.0, v1=3500.0, iface=None):
     return v
 
 
-def _write_traces(fn, path, traces, dt, rec_x, rec_y, src, shot):
+def _write_traces(fn, path, traces, dt, rec_x, rec_y, rec_z, src, shot):
     tr = np.ascontiguousarray(traces, dtype=np.float32)
     nrec, nt = tr.shape
-    rx = None if rec_x is None else np.ascontiguousarray(rec_x, dtype=np.float64)
-    ry = None if rec_y is None else np.ascontiguousarray(rec_y, dtype=np.float64)
-    rc = fn(str(path).encode(), tr.ctypes.data_as(_fp), nrec, nt, dt, None if rx is None else rx.ctypes.data_as(_dp),
-            None if ry is None else ry.ctypes.data_as(_dp), src[0], src[1], shot)
+    coords = [None if c is None else np.ascontiguousarray(c, dtype=np.float64) for c in (rec_x, rec_y, rec_z)]
+    ptrs = [None if c is None else c.ctypes.data_as(_dp) for c in coords]
+    src = tuple(src) + (0.0,) * (3 - len(src))
+    rc = fn(str(path).encode(), tr.ctypes.data_as(_fp), nrec, nt, dt, *ptrs, src[0], src[1], src[2], shot)
     if rc:
         raise OseError(rc, f"cannot write {path}")
 
@@ -122,16 +122,20 @@ def _read_traces(fn, path, cap):
     return buf[: nrec.value * nt.value].reshape(nrec.value, nt.value).copy(), dt.value
 
 
-def write_su(path, traces, dt, rec_x=None, rec_y=None, src=(0.0, 0.0), shot=0):
-    _write_traces(_L().oseh_write_su, path, traces, dt, rec_x, rec_y, src, shot)
+def write_su(path, traces, dt, rec_x=None, rec_y=None, src=(0.0, 0.0, 0.0), shot=0, rec_z=None):
+    """rec_x/rec_y/rec_z: receiver coordinates in metres (z = depth);
+    src: (x, y[, depth])."""
+    _write_traces(_L().oseh_write_su, path, traces, dt, rec_x, rec_y, rec_z, src, shot)
 
 
 def read_su(path, cap=1 << 26):
     return _read_traces(_L().oseh_read_su, path, cap)
 
 
-def write_segy(path, traces, dt, rec_x=None, rec_y=None, src=(0.0, 0.0), shot=0):
-    _write_traces(_L().oseh_write_segy, path, traces, dt, rec_x, rec_y, src, shot)
+def write_segy(path, traces, dt, rec_x=None, rec_y=None, src=(0.0, 0.0, 0.0), shot=0, rec_z=None):
+    """rec_x/rec_y/rec_z: receiver coordinates in metres (z = depth);
+    src: (x, y[, depth])."""
+    _write_traces(_L().oseh_write_segy, path, traces, dt, rec_x, rec_y, rec_z, src, shot)
 
 
 def read_segy(path, cap=1 << 26):
@@ -243,10 +247,17 @@ class Simulation:
             out[n] = (ms.value, cnt.value, by.value)
         return out
 
+    def receivers(self, shot):
+        """Interior (x, y, z) of the shot's receivers, one row each, in trace order."""
+        n = ctypes.c_int(0)
+        self._check(self._L.oseh_sim_receivers(self._h, shot, None, ctypes.byref(n)))
+        out = np.empty((n.value, 3), dtype=np.int32)
+        self._check(self._L.oseh_sim_receivers(self._h, shot, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                               ctypes.byref(n)))
+        return out
+
     def traces(self, shot, it0, nsteps):
-        m = self.shot(shot)["model"]
-        d = self.model_dims(m)
-        out = np.empty((d[0] * d[1], nsteps), dtype=np.float32)
+        out = np.empty((len(self.receivers(shot)), nsteps), dtype=np.float32)
         rc = self._hip.ose_get_traces(self.ctx, shot, it0, nsteps, out.ctypes.data_as(_fp))
         if rc:
             raise OseError(rc, self._hip.ose_last_error(self.ctx).decode())

@@ -85,10 +85,22 @@ int ose_sync(ose_ctx* ctx);
 /* Same as ose_run + ose_sync, timed with HIP events on the context stream. */
 int ose_timed_run(ose_ctx* ctx, int nsteps, double* ms);
 
-/* Receiver traces Im(u) on the plane rec_z over the interior x, y, for steps
- * [it0, it0+nsteps): out is trace-major [iy*nx+ix][nsteps] like the
- * reference's seismogram (rec_shotdata_3d ose_3d_kernel.cu:412-436,
- * rec_slice_to_data ose_3d_lib.cu:876-884, write ose_3d.cu:276).          */
+/* Receivers of a shot at listed padded cells xyz[3r..3r+2] = (x, y, z),
+ * r < n, instead of the plane rec_z.  Replaces the receiver geometry the
+ * reference builds but does not use: gxbeg..gzbeg, jgx..jgz, ng
+ * (ose_3d.cu:100-114) through sg_init_device (ose_3d_kernel.cu:202-212)
+ * and the commented-out cell lookup of rec_shotdata_3d
+ * (ose_3d_kernel.cu:417-423).  Before the first ose_run; n = 0 restores
+ * the plane.  Receivers outside the grid fail with OSE_EINVAL.            */
+int ose_set_receivers(ose_ctx* ctx, int shot_id, const int* xyz, int n);
+/* Number of traces of a shot: its listed receivers, else nx*ny.           */
+int ose_receiver_count(ose_ctx* ctx, int shot_id, int* nrec);
+
+/* Receiver traces Im(u) on the plane rec_z over the interior x, y (or at
+ * the shot's listed receivers, in list order), for steps [it0, it0+nsteps):
+ * out is trace-major [iy*nx+ix][nsteps] like the reference's seismogram
+ * (rec_shotdata_3d ose_3d_kernel.cu:412-436, rec_slice_to_data
+ * ose_3d_lib.cu:876-884, write ose_3d.cu:276).                            */
 int ose_get_traces(ose_ctx* ctx, int shot_id, int it0, int nsteps, float* out);
 /* Upload recorded traces (same layout as ose_get_traces), e.g. when a run
  * resumes from a checkpoint.  No counterpart in the reference.            */

@@ -80,13 +80,15 @@ void oseh_rng_seed(uint64_t* state, uint64_t seed);
  * ose_3d_lib.cu:327-343, called at ose_3d.cu:276).                         */
 int oseh_write_raw(const char* path, const float* data, size_t n);
 /* SU file: per trace a 240-byte SEG-Y trace header (native endianness:
- * tracl @0, tracr @4, fldr @8, tracf @12, ep @16, cdp @20, sx @72, sy @76,
- * gx @80, gy @84, scalco @70, ns @114 (u16), dt @116 (u16, microseconds))
- * followed by nt float32 samples.  Coordinates in metres times 100
- * (scalco = -100).  rec_x/rec_y: receiver coordinates (m), src_x/src_y.   */
+ * tracl @0, tracr @4, fldr @8, tracf @12, ep @16, cdp @20, offset @36,
+ * gelev @40, sdepth @48, scalel @68, scalco @70, sx @72, sy @76, gx @80,
+ * gy @84, ns @114 (u16), dt @116 (u16, microseconds)) followed by nt
+ * float32 samples.  Coordinates and elevations in metres times 100
+ * (scalco = scalel = -100); gelev = -(receiver depth).  rec_x/rec_y/rec_z:
+ * receiver coordinates (m, z = depth; any may be NULL for 0), src_x/y/z.  */
 int oseh_write_su(const char* path, const float* traces, int nrec, int nt, double dt,
-                  const double* rec_x, const double* rec_y, double src_x, double src_y,
-                  int shot_index);
+                  const double* rec_x, const double* rec_y, const double* rec_z, double src_x,
+                  double src_y, double src_z, int shot_index);
 /* Read back an SU file written above: returns nrec, nt and the samples.   */
 int oseh_read_su(const char* path, float* traces, size_t cap, int* nrec, int* nt, double* dt);
 /* SEG-Y rev 1 file: 3200-byte EBCDIC textual header, 400-byte big-endian
@@ -96,8 +98,8 @@ int oseh_read_su(const char* path, float* traces, size_t cap, int* nrec, int* nt
  * IEEE float32 samples.  nt and the interval in microseconds must fit in
  * 15 bits.                                                                 */
 int oseh_write_segy(const char* path, const float* traces, int nrec, int nt, double dt,
-                    const double* rec_x, const double* rec_y, double src_x, double src_y,
-                    int shot_index);
+                    const double* rec_x, const double* rec_y, const double* rec_z, double src_x,
+                    double src_y, double src_z, int shot_index);
 /* Read a SEG-Y file (format 5 IEEE or format 1 IBM floats, fixed trace
  * length): returns nrec, nt, dt and the samples as IEEE float32.          */
 int oseh_read_segy(const char* path, float* traces, size_t cap, int* nrec, int* nt, double* dt);
@@ -114,7 +116,13 @@ int oseh_write_snapshot(const char* path, const float* f_zyx, int nxb, int nyb, 
  * Keys: nx ny nz dx dy dz dt nt fm amp delay boundary bl br bf bb bu bd
  *       model seed v0 v1 iface vfile sx sy sz nshot jsx jsy jsz srcgrid
  *       rec_z korder ktol taper alpha extent hilbert device graph
- *       max_shots (cap on shots per model) snap                            */
+ *       max_shots (cap on shots per model) snap segy
+ * Receivers: by default the whole interior plane rec_z.  ng=<n> records a
+ * line (gxbeg + r*jgx, gybeg + r*jgy, gzbeg + r*jgz), r < n (the
+ * reference's gxbeg..jgz, ng, ose_3d.cu:100-114; defaults jgx = 1,
+ * jgy = jgz = 0); ngx/ngy record an ngx x ngy grid at depth gzbeg with
+ * spacing jgx, jgy (default 1); csdgather=1 shifts every shot's receivers
+ * by its source's offset from the model's first source.  Interior cells. */
 typedef struct oseh_sim oseh_sim;
 int  oseh_sim_create(oseh_sim** out, const char* params);
 /* Build velocities, pads, wavelets, orders and coefficients (host only).  */
@@ -133,6 +141,11 @@ int oseh_read_params(const char* path, float* out, size_t cap, size_t* n);
 /* JSON description: models (interior, padded, K, R, taper), shots,
  * interior and padded cells per step.                                      */
 int  oseh_sim_info(oseh_sim* sim, char* buf, size_t cap);
+/* Receivers of shot k (after oseh_sim_build): interior (x, y, z) triples,
+ * the plane rec_z (x fastest) unless a line or grid is configured.  *n is
+ * the capacity of xyz in receivers on entry (xyz may be NULL to query) and
+ * the count on return; OSE_EINVAL if xyz is too small.                    */
+int  oseh_sim_receivers(oseh_sim* sim, int shot, int* xyz, int* n);
 /* Write traces (raw + .su) and, if requested, snapshots into dir.          */
 int  oseh_sim_write(oseh_sim* sim, const char* dir);
 /* Write the current wavefield of every shot as p-<it>-shot<k>.bin (Re u)

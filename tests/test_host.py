@@ -124,6 +124,15 @@ def test_su_round_trip_and_header(H, tmp_path):
     assert struct.unpack_from("<i", h, 80)[0] == 1000      # gx = 10 m * 100
     assert struct.unpack_from("<H", h, 114)[0] == 37       # ns
     assert struct.unpack_from("<H", h, 116)[0] == 1000     # dt (us)
+    assert struct.unpack_from("<i", h, 40)[0] == 0         # gelev: no depth given
+    # receiver depths and the source depth: gelev = -depth, sdepth, scalel
+    H.write_su(p, tr, 1e-3, rec_x=np.arange(6) * 10.0, rec_y=np.zeros(6), rec_z=np.arange(6) * 5.0,
+               src=(25.0, 0.0, 30.0), shot=2)
+    raw = open(p, "rb").read()
+    h = raw[3 * (240 + 4 * 37):][:240]   # fourth trace
+    assert struct.unpack_from("<i", h, 40)[0] == -1500     # gelev = -(15 m) * 100
+    assert struct.unpack_from("<i", h, 48)[0] == 3000      # sdepth = 30 m * 100
+    assert struct.unpack_from("<h", h, 68)[0] == -100      # scalel
 
 
 def test_snapshot_order(H, tmp_path):
@@ -235,3 +244,45 @@ def test_segy_reads_ibm_floats(H, tmp_path):
     got, dt = H.read_segy(p)
     assert got.shape == (1, len(vals)) and abs(dt - 1e-3) < 1e-12
     assert np.allclose(got[0], list(vals.values()), rtol=0, atol=1e-6)
+
+
+def _receivers(H, params):
+    sim = H.Simulation(params).build()
+    out = [sim.receivers(k) for k in range(sim.nshots())]
+    sim.close()
+    return out
+
+
+def test_receiver_geometry(H):
+    """Receiver line and grid from gxbeg..jgz / ng / ngx, ngy, csdgather
+    (ose_3d.cu:100-114 and sg_init_device ose_3d_kernel.cu:202-212: receiver
+    r at (gxbeg + r*jgx, gybeg + r*jgy, gzbeg + r*jgz))."""
+    base = "nx=20 ny=16 nz=12 boundary=4 nt=4 model=const extent=smooth "
+    # default: the interior plane rec_z, x fastest
+    (plane,) = _receivers(H, base + "rec_z=3")
+    assert plane.shape == (20 * 16, 3)
+    assert np.array_equal(plane[21], [1, 1, 3]) and (plane[:, 2] == 3).all()
+    # a line exactly as sg_init_device lays it out
+    (line,) = _receivers(H, base + "ng=5 gxbeg=2 gybeg=3 gzbeg=1 jgx=3 jgy=2 jgz=2")
+    r = np.arange(5)
+    assert np.array_equal(line, np.stack([2 + 3 * r, 3 + 2 * r, 1 + 2 * r], axis=1))
+    # defaults of a line: along x at the surface
+    (line,) = _receivers(H, base + "ng=4 gxbeg=1")
+    assert np.array_equal(line, [[1, 0, 0], [2, 0, 0], [3, 0, 0], [4, 0, 0]])
+    # a grid at depth gzbeg, x fastest
+    (grid,) = _receivers(H, base + "ngx=3 ngy=2 gxbeg=1 gybeg=2 gzbeg=4 jgx=2 jgy=5")
+    assert np.array_equal(grid, [[1, 2, 4], [3, 2, 4], [5, 2, 4], [1, 7, 4], [3, 7, 4], [5, 7, 4]])
+    # three shots along x; fixed spread, then the spread moving with the source
+    shots = base + "nshot=3 sx=4 sy=8 jsx=3 jsy=0 jsz=0 ng=4 gxbeg=2 gybeg=8 "
+    fixed = _receivers(H, shots)
+    assert all(np.array_equal(f, fixed[0]) for f in fixed)
+    moving = _receivers(H, shots + "csdgather=1")
+    for k, m in enumerate(moving):
+        assert np.array_equal(m, fixed[0] + [3 * k, 0, 0])
+
+
+@pytest.mark.parametrize("bad", ["ng=30", "ng=3 gzbeg=12", "ngx=2 gxbeg=-1", "ng=2 ngx=2", "ng=-1",
+                                 "nshot=3 sx=4 jsx=6 ng=4 gxbeg=10 csdgather=1"])
+def test_receiver_geometry_rejects_cells_outside(H, bad):
+    with pytest.raises(H.OseError):
+        H.Simulation("nx=20 ny=16 nz=12 boundary=4 nt=4 model=const extent=smooth " + bad).build()
