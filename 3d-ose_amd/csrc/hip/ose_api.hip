@@ -12,12 +12,16 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <random>
+#include <thread>
 #include <algorithm>
 #include <string>
 #include <vector>
@@ -26,6 +30,7 @@
 #include "ose_bnet.h"
 #include "ose_entry.h"
 #include "ose_kernels.h"
+#include "ose_snap.h"
 
 namespace ose {
 
@@ -233,6 +238,39 @@ struct Pending {
     double bytes;
 };
 
+// Snapshot streaming: every `every` steps each shot's state is encoded on
+// the compute stream into a free device slot (the enqueuing thread waits for
+// one only when the host side has fallen that far behind).  A copier thread
+// moves each encoded snapshot into a free pinned host slot on a second
+// stream and frees the device slot; a writer thread hands it to the callback
+// and frees the host slot.  There are as many device slots as shots (so one
+// snapshot of every shot never waits) within half of the free device memory,
+// and three host slots.
+struct SnapStream {
+    static constexpr int kHost = 3;
+    int every = 0, lanes = 3, format = SNAP_F32, interior = 0;
+    ose_snapshot_fn fn = nullptr;
+    void* user = nullptr;
+    bool started = false;
+    size_t slot_bytes = 0;
+    std::vector<std::unique_ptr<DevBuf>> dev;
+    std::vector<hipEvent_t> ready;
+    std::vector<char> dev_busy;
+    void* host[kHost] = {};
+    hipEvent_t copied[kHost] = {};
+    bool host_busy[kHost] = {};
+    hipStream_t copy = nullptr;
+    int next = 0;
+    std::thread copier, writer;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<std::pair<int, ose_snapshot_info>> to_copy, to_write;
+    bool stop = false, copier_done = false;
+    long enqueued = 0, delivered = 0;
+    int err = OSE_OK;
+    std::string msg;
+};
+
 }  // namespace
 
 struct ose_ctx {
@@ -255,6 +293,9 @@ struct ose_ctx {
     double st_ms[PK_N] = {};
     long st_n[PK_N] = {};
     double st_bytes[PK_N] = {};
+    // snapshot streaming and the host's count of enqueued steps
+    std::unique_ptr<SnapStream> snap;
+    int host_it = 0;
 
     int fail(int code, const char* fmt, ...)
     {
@@ -789,6 +830,224 @@ Shot* get_shot(ose_ctx* c, int id)
     return &c->shots[id];
 }
 
+// ---- snapshot streaming ---------------------------------------------------
+
+void snap_box(const SnapStream& s, const Model& m, int* o, int* n)
+{
+    if (s.interior) {
+        o[0] = m.d.bl, o[1] = m.d.bf, o[2] = m.d.bu;
+        n[0] = m.d.nx, n[1] = m.d.ny, n[2] = m.d.nz;
+    } else {
+        o[0] = o[1] = o[2] = 0;
+        n[0] = m.d.nxb, n[1] = m.d.nyb, n[2] = m.d.nzb;
+    }
+}
+
+int snap_lanes(int lanes) { return (lanes & 1) + ((lanes >> 1) & 1); }
+
+void snap_fail(SnapStream* s, const char* what, hipError_t e)
+{
+    if (s->err == OSE_OK) {
+        s->err = OSE_EDEVICE;
+        s->msg = std::string(what) + ": " + hipGetErrorString(e);
+    }
+}
+
+void snap_copier(int device, SnapStream* s)
+{
+    (void)hipSetDevice(device);
+    for (;;) {
+        std::pair<int, ose_snapshot_info> job;
+        int h = -1;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv.wait(lk, [s] { return s->stop || !s->to_copy.empty(); });
+            if (s->to_copy.empty()) break;
+            job = s->to_copy.front();
+            s->to_copy.pop_front();
+            s->cv.wait(lk, [s, &h] {
+                for (int k = 0; k < SnapStream::kHost; ++k)
+                    if (!s->host_busy[k]) return (h = k), true;
+                return false;
+            });
+            s->host_busy[h] = true;
+        }
+        const int d = job.first;
+        hipError_t e = hipStreamWaitEvent(s->copy, s->ready[d], 0);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(s->host[h], s->dev[d]->p, job.second.bytes, hipMemcpyDeviceToHost, s->copy);
+        if (e == hipSuccess) e = hipEventRecord(s->copied[h], s->copy);
+        if (e == hipSuccess) e = hipEventSynchronize(s->copied[h]);
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->dev_busy[d] = 0;
+            if (e == hipSuccess) {
+                s->to_write.emplace_back(h, job.second);
+            } else {
+                snap_fail(s, "snapshot copy failed", e);
+                s->host_busy[h] = false;
+                ++s->delivered;
+            }
+        }
+        s->cv.notify_all();
+    }
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->copier_done = true;
+    }
+    s->cv.notify_all();
+}
+
+void snap_writer(SnapStream* s)
+{
+    for (;;) {
+        std::pair<int, ose_snapshot_info> job;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv.wait(lk, [s] { return s->copier_done || !s->to_write.empty(); });
+            if (s->to_write.empty()) return;
+            job = s->to_write.front();
+            s->to_write.pop_front();
+        }
+        s->fn(s->user, &job.second, s->host[job.first]);
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->host_busy[job.first] = false;
+            ++s->delivered;
+        }
+        s->cv.notify_all();
+    }
+}
+
+// slots sized for the largest model's snapshot, the copy stream and the threads
+int snap_start(ose_ctx* c)
+{
+    SnapStream& s = *c->snap;
+    if (s.started) return OSE_OK;
+    size_t need = 256;
+    for (auto& mp : c->models) {
+        int o[3], n[3];
+        snap_box(s, *mp, o, n);
+        need = std::max(need, snap_bytes(s.format, snap_lanes(s.lanes), n[0], n[1], n[2]));
+    }
+    s.slot_bytes = need;
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(c, hipMemGetInfo(&free_b, &total_b));
+    const size_t fit = free_b / 2 / need;
+    const int nd = (int)std::max<size_t>(1, std::min<size_t>(c->shots.size(), fit));
+    int rc;
+    for (int i = 0; i < nd; ++i) {
+        s.dev.emplace_back(new DevBuf());
+        if ((rc = dev_alloc(c, *s.dev.back(), need))) return rc;
+        hipEvent_t e = nullptr;
+        HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        s.ready.push_back(e);
+        s.dev_busy.push_back(0);
+    }
+    for (int i = 0; i < SnapStream::kHost; ++i) {
+        HIPCHK(c, hipHostMalloc(&s.host[i], need, hipHostMallocDefault));
+        HIPCHK(c, hipEventCreateWithFlags(&s.copied[i], hipEventDisableTiming));
+    }
+    HIPCHK(c, hipStreamCreateWithFlags(&s.copy, hipStreamNonBlocking));
+    s.copier = std::thread(snap_copier, c->device, &s);
+    s.writer = std::thread(snap_writer, &s);
+    s.started = true;
+    return OSE_OK;
+}
+
+int snap_drain(ose_ctx* c)
+{
+    if (!c->snap || !c->snap->started) return OSE_OK;
+    SnapStream& s = *c->snap;
+    std::unique_lock<std::mutex> lk(s.mu);
+    s.cv.wait(lk, [&s] { return s.delivered == s.enqueued; });
+    if (s.err != OSE_OK) return c->fail(s.err, "%s", s.msg.c_str());
+    return OSE_OK;
+}
+
+// drain, stop the threads and release the slots
+int snap_stop(ose_ctx* c)
+{
+    if (!c->snap) return OSE_OK;
+    SnapStream& s = *c->snap;
+    int rc = OSE_OK;
+    if (s.started) rc = snap_drain(c);
+    {
+        std::lock_guard<std::mutex> lk(s.mu);
+        s.stop = true;
+    }
+    s.cv.notify_all();
+    if (s.copier.joinable()) s.copier.join();
+    if (s.writer.joinable()) s.writer.join();
+    for (auto e : s.ready)
+        if (e) (void)hipEventDestroy(e);
+    for (int i = 0; i < SnapStream::kHost; ++i) {
+        if (s.host[i]) (void)hipHostFree(s.host[i]);
+        if (s.copied[i]) (void)hipEventDestroy(s.copied[i]);
+    }
+    if (s.copy) (void)hipStreamDestroy(s.copy);
+    c->snap.reset();
+    return rc;
+}
+
+// encode and stream the state of every shot after `step` steps
+int snap_enqueue(ose_ctx* c, int step)
+{
+    SnapStream& s = *c->snap;
+    for (auto& mp : c->models) {
+        Model& m = *mp;
+        for (int id : m.shots) {
+            const Shot& sh = c->shots[id];
+            const int i = s.next;
+            s.next = (i + 1) % (int)s.dev.size();
+            {
+                std::unique_lock<std::mutex> lk(s.mu);
+                s.cv.wait(lk, [&s, i] { return !s.dev_busy[i] || s.err != OSE_OK; });
+                if (s.err != OSE_OK) return c->fail(s.err, "%s", s.msg.c_str());
+                s.dev_busy[i] = 1;
+            }
+            int o[3], n[3];
+            snap_box(s, m, o, n);
+            SnapArgs a{};
+            a.u = m.W[0] + (long)sh.slot * m.ncell;
+            a.nxb = m.d.nxb;
+            a.nyb = m.d.nyb;
+            a.x0 = o[0], a.y0 = o[1], a.z0 = o[2];
+            a.nx = n[0], a.ny = n[1], a.nz = n[2];
+            a.lanes = s.lanes;
+            a.format = s.format;
+            a.out = s.dev[i]->p;
+            const void* fn = s.format == SNAP_F32    ? (const void*)&k_snap<SNAP_F32>
+                             : s.format == SNAP_BF16 ? (const void*)&k_snap<SNAP_BF16>
+                             : s.format == SNAP_Q16  ? (const void*)&k_snap<SNAP_Q16>
+                                                     : (const void*)&k_snap<SNAP_Q8>;
+            ose_snapshot_info info{};
+            info.shot_id = id;
+            info.step = step;
+            info.lanes = s.lanes;
+            info.format = s.format;
+            info.x0 = o[0], info.y0 = o[1], info.z0 = o[2];
+            info.nx = n[0], info.ny = n[1], info.nz = n[2];
+            info.bytes = snap_bytes(s.format, snap_lanes(s.lanes), n[0], n[1], n[2]);
+            int rc = launch(c, -1, 0.0, fn, dim3(n[1], n[2]), dim3(kSnapThreads), 0, &a);
+            hipError_t e = hipSuccess;
+            if (!rc) e = hipEventRecord(s.ready[i], c->stream);
+            if (rc || e != hipSuccess) {
+                std::lock_guard<std::mutex> lk(s.mu);
+                s.dev_busy[i] = 0;
+                return rc ? rc : c->fail(OSE_EDEVICE, "snapshot enqueue failed: %s", hipGetErrorString(e));
+            }
+            {
+                std::lock_guard<std::mutex> lk(s.mu);
+                s.to_copy.emplace_back(i, info);
+                ++s.enqueued;
+            }
+            s.cv.notify_all();
+        }
+    }
+    return OSE_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------
@@ -830,6 +1089,7 @@ int ose_destroy(ose_ctx* c)
     if (!c) return OSE_EINVAL;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    (void)snap_stop(c);
     drop_graph(c);
     for (auto& p : c->pending) {
         c->ev_pool.push_back(p.e0);
@@ -1025,12 +1285,16 @@ int ose_run(ose_ctx* c, int nsteps)
         c->use_graph = false;
         drop_graph(c);
     }
+    const bool snaps = c->snap && c->snap->every > 0;
+    if (snaps && (rc = snap_start(c))) return rc;
     for (int s = 0; s < nsteps; ++s) {
         if (c->use_graph && !c->profile && c->gexec) {
             HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
         } else if ((rc = enqueue_step(c))) {
             return rc;
         }
+        ++c->host_it;
+        if (snaps && c->host_it % c->snap->every == 0 && (rc = snap_enqueue(c, c->host_it))) return rc;
     }
     return OSE_OK;
 }
@@ -1041,7 +1305,37 @@ int ose_sync(ose_ctx* c)
     int rc;
     if ((rc = set_device(c))) return rc;
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((rc = snap_drain(c))) return rc;
     return collect(c);
+}
+
+int ose_set_snapshots(ose_ctx* c, int every, int lanes, int format, int interior, ose_snapshot_fn fn, void* user)
+{
+    if (!c) return OSE_EINVAL;
+    if (every < 0 || (every > 0 && (!fn || lanes < 1 || lanes > 3 || format < OSE_SNAP_F32 || format > OSE_SNAP_Q8)))
+        return c->fail(OSE_EINVAL, "ose_set_snapshots: bad cadence %d, lanes %d, format %d or callback", every, lanes,
+                       format);
+    int rc;
+    if ((rc = set_device(c))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((rc = snap_stop(c))) return rc;
+    if (every == 0) return OSE_OK;
+    c->snap.reset(new (std::nothrow) SnapStream());
+    if (!c->snap) return c->fail(OSE_ENOMEM, "ose_set_snapshots: out of memory");
+    SnapStream& s = *c->snap;
+    s.every = every;
+    s.lanes = lanes;
+    s.format = format;
+    s.interior = interior != 0;
+    s.fn = fn;
+    s.user = user;
+    return OSE_OK;
+}
+
+int ose_snapshot_drain(ose_ctx* c)
+{
+    if (!c) return OSE_EINVAL;
+    return snap_drain(c);
 }
 
 int ose_timed_run(ose_ctx* c, int nsteps, double* ms)
@@ -1262,6 +1556,7 @@ int ose_set_step(ose_ctx* c, int it)
     int rc;
     if ((rc = ose_sync(c))) return rc;
     HIPCHK(c, hipMemcpy(c->d_it, &it, sizeof(int), hipMemcpyHostToDevice));
+    c->host_it = it;
     return OSE_OK;
 }
 

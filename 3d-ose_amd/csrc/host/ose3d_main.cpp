@@ -3,7 +3,7 @@
 // SU-style key=value arguments and presets.
 //
 //   ose3d preset=ref [vfile=input/v3d_nx101_ny101_nz101.bin] out=output
-//   ose3d preset=c1 nt=200 out=/tmp/run snap_every=50
+//   ose3d preset=c1 nt=200 out=/tmp/run snap_every=50 [snap_format=q8 snap_region=interior snap_lanes=q]
 //   ose3d preset=c4 checkpoint=/scratch/ck checkpoint_every=500 [restore=/scratch/ck]
 //
 // Prints one JSON line with the model set-up and the measured rate.
@@ -24,8 +24,9 @@ int main(int argc, char** argv)
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
         if (a == "-h" || a == "--help") {
-            std::printf("usage: ose3d [preset=ref|c0|c1|c2|c3|c4] [key=value ...] [out=dir] [snap_every=N]\n"
-                        "             [checkpoint=dir checkpoint_every=N] [restore=dir]\n");
+            std::printf("usage: ose3d [preset=ref|c0|c1|c2|c3|c4] [key=value ...] [out=dir]\n"
+                        "             [snap_every=N snap_format=f32|bf16|q16|q8 snap_region=padded|interior\n"
+                        "              snap_lanes=pq|p|q] [checkpoint=dir checkpoint_every=N] [restore=dir]\n");
             return 0;
         }
         if (a.rfind("out=", 0) == 0) out = a.substr(4);
@@ -70,9 +71,14 @@ int main(int argc, char** argv)
     }
     const int first = done;
     if (!ckpt.empty()) mkdir(ckpt.c_str(), 0755);
+    // snapshots stream from the library while the steps run
+    if (snap_every > 0 && (rc = oseh_sim_stream_snapshots(sim, out.c_str(), snap_every))) {
+        std::fprintf(stderr, "ose3d: snapshots: %s\n", oseh_sim_error(sim));
+        oseh_sim_destroy(sim);
+        return 1;
+    }
     while (done < nt) {
         int chunk = nt - done;
-        if (snap_every > 0) chunk = std::min(chunk, snap_every - done % snap_every);
         if (ckpt_every > 0) chunk = std::min(chunk, ckpt_every - done % ckpt_every);
         double ms = 0.0;
         if ((rc = oseh_sim_run(sim, chunk, &ms))) {
@@ -82,11 +88,6 @@ int main(int argc, char** argv)
         }
         total_ms += ms;
         done += chunk;
-        if (snap_every > 0 && done % snap_every == 0 && (rc = oseh_sim_snapshot(sim, out.c_str()))) {
-            std::fprintf(stderr, "ose3d: snapshot failed: %s\n", oseh_sim_error(sim));
-            oseh_sim_destroy(sim);
-            return 1;
-        }
         if (!ckpt.empty() && ckpt_every > 0 && done % ckpt_every == 0 && done < nt &&
             (rc = oseh_sim_checkpoint(sim, ckpt.c_str()))) {
             std::fprintf(stderr, "ose3d: checkpoint failed: %s\n", oseh_sim_error(sim));

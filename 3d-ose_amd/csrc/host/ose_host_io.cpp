@@ -309,4 +309,122 @@ int oseh_write_snapshot(const char* path, const float* f_zyx, int nxb, int nyb, 
     return (ok && rc == 0) ? OSE_OK : OSE_EINVAL;
 }
 
+// ---- encoded snapshots (the layouts of ose_set_snapshots) -----------------
+
+static int snap_vbytes(int format) { return format == OSE_SNAP_F32 ? 4 : format == OSE_SNAP_Q8 ? 1 : 2; }
+static bool snap_scaled(int format) { return format == OSE_SNAP_Q16 || format == OSE_SNAP_Q8; }
+
+size_t oseh_snapshot_lane_bytes(int format, int nx, int ny, int nz)
+{
+    if (format < OSE_SNAP_F32 || format > OSE_SNAP_Q8 || nx <= 0 || ny <= 0 || nz <= 0) return 0;
+    const size_t cells = (size_t)nx * ny * nz;
+    return (snap_scaled(format) ? sizeof(float) * (size_t)ny * nz : 0) + cells * snap_vbytes(format);
+}
+
+int oseh_snapshot_lane(int format, int nl, int lane, int nx, int ny, int nz, const void* data, void* out)
+{
+    if (!data || !out || nl < 1 || nl > 2 || lane < 0 || lane >= nl) return OSE_EINVAL;
+    const size_t lb = oseh_snapshot_lane_bytes(format, nx, ny, nz);
+    if (!lb) return OSE_EINVAL;
+    const size_t cells = (size_t)nx * ny * nz, lines = (size_t)ny * nz;
+    const char* d = (const char*)data;
+    char* o = (char*)out;
+    size_t vo = 0;
+    if (snap_scaled(format)) {
+        std::memcpy(o, d + sizeof(float) * lines * lane, sizeof(float) * lines);
+        o += sizeof(float) * lines;
+        vo = sizeof(float) * lines * nl;
+    }
+    const size_t vb = cells * snap_vbytes(format);
+    std::memcpy(o, d + vo + vb * lane, vb);
+    return OSE_OK;
+}
+
+int oseh_decode_snapshot(int format, int nx, int ny, int nz, const void* lane_data, float* out)
+{
+    if (!lane_data || !out || !oseh_snapshot_lane_bytes(format, nx, ny, nz)) return OSE_EINVAL;
+    const size_t cells = (size_t)nx * ny * nz, lines = (size_t)ny * nz;
+    const char* d = (const char*)lane_data;
+    if (format == OSE_SNAP_F32) {
+        std::memcpy(out, d, sizeof(float) * cells);
+    } else if (format == OSE_SNAP_BF16) {
+        for (size_t i = 0; i < cells; ++i) {
+            uint16_t h;
+            std::memcpy(&h, d + 2 * i, 2);
+            const uint32_t b = (uint32_t)h << 16;
+            std::memcpy(out + i, &b, 4);
+        }
+    } else {
+        const float* sc = (const float*)d;
+        const float qmax = format == OSE_SNAP_Q8 ? 127.f : 32767.f;
+        const char* v = d + sizeof(float) * lines;
+        for (size_t l = 0; l < lines; ++l) {
+            const float step = sc[l] / qmax;
+            for (int x = 0; x < nx; ++x) {
+                const size_t i = l * nx + x;
+                float q;
+                if (format == OSE_SNAP_Q8) {
+                    q = (float)(int8_t)v[i];
+                } else {
+                    int16_t t;
+                    std::memcpy(&t, v + 2 * i, 2);
+                    q = (float)t;
+                }
+                out[i] = q * step;
+            }
+        }
+    }
+    return OSE_OK;
+}
+
+// .osnap: 64-byte little-endian header, then one lane's encoded data
+//   0 "OSESNAP1"  8 format  12 lane (0 P, 1 Q)  16 step  20 shot
+//   24 x0 y0 z0  36 nx ny nz  48 u64 payload bytes  56 reserved
+int oseh_write_osnap(const char* path, int format, int lane, int step, int shot, const int* box,
+                     const void* lane_data)
+{
+    if (!path || !box || !lane_data) return OSE_EINVAL;
+    const size_t pb = oseh_snapshot_lane_bytes(format, box[3], box[4], box[5]);
+    if (!pb) return OSE_EINVAL;
+    unsigned char h[64] = {};
+    std::memcpy(h, "OSESNAP1", 8);
+    const int32_t f[10] = {format, lane, step, shot, box[0], box[1], box[2], box[3], box[4], box[5]};
+    std::memcpy(h + 8, f, sizeof f);
+    const uint64_t n = pb;
+    std::memcpy(h + 48, &n, 8);
+    FILE* fp = std::fopen(path, "wb");
+    if (!fp) return OSE_EINVAL;
+    const bool ok = std::fwrite(h, 1, 64, fp) == 64 && std::fwrite(lane_data, 1, pb, fp) == pb;
+    const int rc = std::fclose(fp);
+    return (ok && rc == 0) ? OSE_OK : OSE_EINVAL;
+}
+
+int oseh_read_osnap(const char* path, float* out, size_t cap, int* meta)
+{
+    if (!path || !meta) return OSE_EINVAL;
+    FILE* fp = std::fopen(path, "rb");
+    if (!fp) return OSE_EINVAL;
+    unsigned char h[64];
+    int rc = OSE_OK;
+    int32_t f[10] = {};
+    uint64_t n = 0;
+    if (std::fread(h, 1, 64, fp) != 64 || std::memcmp(h, "OSESNAP1", 8) != 0) rc = OSE_EINVAL;
+    if (!rc) {
+        std::memcpy(f, h + 8, sizeof f);
+        std::memcpy(&n, h + 48, 8);
+        if (n != oseh_snapshot_lane_bytes(f[0], f[7], f[8], f[9]) || n == 0) rc = OSE_EINVAL;
+    }
+    if (!rc) {
+        std::memcpy(meta, f, sizeof f);
+        const size_t cells = (size_t)f[7] * f[8] * f[9];
+        if (out) {
+            std::vector<char> buf(n);
+            if (cap < cells || std::fread(buf.data(), 1, n, fp) != n || std::fgetc(fp) != EOF) rc = OSE_EINVAL;
+            else rc = oseh_decode_snapshot(f[0], f[7], f[8], f[9], buf.data(), out);
+        }
+    }
+    std::fclose(fp);
+    return rc;
+}
+
 }  // extern "C"

@@ -10,6 +10,7 @@
 #include <fstream>
 #include <iterator>
 #include <map>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -58,6 +59,10 @@ struct oseh_sim {
     ose_ctx* ctx = nullptr;
     std::string err;
     int steps_done = 0;
+    // streamed snapshots: output directory and the first write error
+    std::string snap_dir;
+    std::mutex snap_mu;
+    std::string snap_err;
 
     int fail(int code, const std::string& m)
     {
@@ -80,6 +85,13 @@ struct oseh_sim {
         return it == p.end() ? d : std::atoi(it->second.c_str());
     }
 };
+
+// first failed write of a streamed snapshot, reported once the run returns
+static int snap_error(oseh_sim* s)
+{
+    std::lock_guard<std::mutex> lk(s->snap_mu);
+    return s->snap_err.empty() ? OSE_OK : s->fail(OSE_EINVAL, s->snap_err);
+}
 
 namespace {
 
@@ -497,7 +509,7 @@ int oseh_sim_run(oseh_sim* s, int nsteps, double* ms)
     if (rc) return s->fail(rc, std::string("ose_timed_run: ") + ose_last_error(s->ctx));
     s->steps_done += nsteps;
     if (ms) *ms = t;
-    return OSE_OK;
+    return snap_error(s);
 }
 
 int oseh_sim_info(oseh_sim* s, char* buf, size_t cap)
@@ -559,9 +571,11 @@ int oseh_sim_write(oseh_sim* s, const char* dir)
 {
     if (!s || !dir) return OSE_EINVAL;
     if (!s->ctx) return s->fail(OSE_ESTATE, "not set up");
+    int rc;
+    if ((rc = ose_sync(s->ctx))) return s->fail(rc, std::string("ose_sync: ") + ose_last_error(s->ctx));
+    if ((rc = snap_error(s))) return rc;
     const int nt = std::min(s->steps_done, s->geti("nt", 300));
     const double dx = s->getd("dx", 10), dy = s->getd("dy", 10), dz = s->getd("dz", 10), dt = s->getd("dt", 1e-3);
-    int rc;
     for (size_t k = 0; k < s->shots.size(); ++k) {
         const BuiltModel& m = s->models[s->shots[k].model];
         const std::vector<int> cells = receiver_cells(s, k);
@@ -596,6 +610,54 @@ void oseh_sim_destroy(oseh_sim* s)
     if (!s) return;
     if (s->ctx) ose_destroy(s->ctx);
     delete s;
+}
+
+// Writer of streamed snapshots (runs on the library's snapshot thread): per
+// lane p-<step>-shot<k>.bin in the reference's [y][x][z] order for float32,
+// else p-<step>-shot<k>.osnap with the encoded lane.
+static void sim_snapshot_cb(void* user, const ose_snapshot_info* in, const void* data)
+{
+    auto* s = static_cast<oseh_sim*>(user);
+    const int nl = (in->lanes & 1) + ((in->lanes >> 1) & 1);
+    std::vector<char> lane(oseh_snapshot_lane_bytes(in->format, in->nx, in->ny, in->nz));
+    int l = 0;
+    for (int b = 0; b < 2; ++b) {
+        if (!((in->lanes >> b) & 1)) continue;
+        const std::string base = s->snap_dir + "/" + (b == 0 ? "p-" : "q-") + std::to_string(in->step) + "-shot" +
+                                 std::to_string(in->shot_id);
+        int rc = oseh_snapshot_lane(in->format, nl, l++, in->nx, in->ny, in->nz, data, lane.data());
+        if (!rc && in->format == OSE_SNAP_F32) {
+            rc = oseh_write_snapshot((base + ".bin").c_str(), (const float*)lane.data(), in->nx, in->ny, in->nz);
+        } else if (!rc) {
+            const int box[6] = {in->x0, in->y0, in->z0, in->nx, in->ny, in->nz};
+            rc = oseh_write_osnap((base + ".osnap").c_str(), in->format, b, in->step, in->shot_id, box, lane.data());
+        }
+        if (rc) {
+            std::lock_guard<std::mutex> lk(s->snap_mu);
+            if (s->snap_err.empty()) s->snap_err = "cannot write snapshot " + base;
+        }
+    }
+}
+
+int oseh_sim_stream_snapshots(oseh_sim* s, const char* dir, int every)
+{
+    if (!s || every < 0 || (every > 0 && !dir)) return OSE_EINVAL;
+    if (!s->ctx) return s->fail(OSE_ESTATE, "not set up");
+    const std::string f = s->get("snap_format", "f32"), r = s->get("snap_region", "padded"),
+                      l = s->get("snap_lanes", "pq");
+    const int format = f == "f32" ? OSE_SNAP_F32 : f == "bf16" ? OSE_SNAP_BF16 : f == "q16" ? OSE_SNAP_Q16
+                                                                 : f == "q8" ? OSE_SNAP_Q8 : -1;
+    const int lanes = l == "pq" ? 3 : l == "p" ? 1 : l == "q" ? 2 : 0;
+    const int interior = r == "interior" ? 1 : r == "padded" ? 0 : -1;
+    if (format < 0 || lanes == 0 || interior < 0)
+        return s->fail(OSE_EINVAL, "snap_format must be f32|bf16|q16|q8, snap_lanes pq|p|q, snap_region padded|interior");
+    int rc;
+    if ((rc = ose_set_snapshots(s->ctx, 0, 0, 0, 0, nullptr, nullptr)))
+        return s->fail(rc, std::string("ose_set_snapshots: ") + ose_last_error(s->ctx));
+    s->snap_dir = every > 0 ? dir : "";
+    if (every > 0 && (rc = ose_set_snapshots(s->ctx, every, lanes, format, interior, sim_snapshot_cb, s)))
+        return s->fail(rc, std::string("ose_set_snapshots: ") + ose_last_error(s->ctx));
+    return snap_error(s);
 }
 
 int oseh_sim_snapshot(oseh_sim* s, const char* dir)

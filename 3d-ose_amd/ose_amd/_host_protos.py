@@ -29,6 +29,12 @@ PROTOS = {
     "oseh_write_segy": ([cp, fp, c_int, c_int, c_double, dp, dp, dp, c_double, c_double, c_double, c_int], c_int),
     "oseh_read_segy": ([cp, fp, c_size_t, ip, ip, dp], c_int),
     "oseh_write_snapshot": ([cp, fp, c_int, c_int, c_int], c_int),
+    "oseh_snapshot_lane_bytes": ([c_int, c_int, c_int, c_int], c_size_t),
+    "oseh_snapshot_lane": ([c_int, c_int, c_int, c_int, c_int, c_int, vp, vp], c_int),
+    "oseh_decode_snapshot": ([c_int, c_int, c_int, c_int, vp, fp], c_int),
+    "oseh_write_osnap": ([cp, c_int, c_int, c_int, c_int, ip, vp], c_int),
+    "oseh_read_osnap": ([cp, fp, c_size_t, ip], c_int),
+    "oseh_sim_stream_snapshots": ([vp, cp, c_int], c_int),
     "oseh_sim_create": ([P(vp), cp], c_int),
     "oseh_sim_build": ([vp], c_int),
     "oseh_sim_setup": ([vp], c_int),

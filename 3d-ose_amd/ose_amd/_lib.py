@@ -38,6 +38,18 @@ class ModelDesc(ctypes.Structure):
     ]
 
 
+class SnapshotInfo(ctypes.Structure):
+    _fields_ = [
+        ("shot_id", c_int), ("step", c_int), ("lanes", c_int), ("format", c_int),
+        ("x0", c_int), ("y0", c_int), ("z0", c_int), ("nx", c_int), ("ny", c_int), ("nz", c_int),
+        ("bytes", ctypes.c_size_t),
+    ]
+
+
+SNAPSHOT_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, P(SnapshotInfo), ctypes.c_void_p)
+SNAP_FORMATS = {"f32": 0, "bf16": 1, "q16": 2, "q8": 3}
+
+
 def _proto(lib, name, args, res=c_int):
     f = getattr(lib, name)
     f.argtypes = args
@@ -65,6 +77,8 @@ def hip():
         _proto(L, "ose_set_traces", [vp, c_int, c_int, c_int, fp])
         _proto(L, "ose_set_receivers", [vp, c_int, ip, c_int])
         _proto(L, "ose_receiver_count", [vp, c_int, ip])
+        _proto(L, "ose_set_snapshots", [vp, c_int, c_int, c_int, c_int, SNAPSHOT_FN, vp])
+        _proto(L, "ose_snapshot_drain", [vp])
         _proto(L, "ose_get_wavefield", [vp, c_int, fp, fp])
         _proto(L, "ose_set_wavefield", [vp, c_int, fp, fp])
         _proto(L, "ose_get_step", [vp, ip])
@@ -116,7 +130,7 @@ def hip_symbols():
     return [
         "ose_create", "ose_destroy", "ose_last_error", "ose_add_model", "ose_add_shot",
         "ose_run", "ose_sync", "ose_timed_run", "ose_get_traces", "ose_set_traces", "ose_set_receivers",
-        "ose_receiver_count", "ose_get_wavefield",
+        "ose_receiver_count", "ose_set_snapshots", "ose_snapshot_drain", "ose_get_wavefield",
         "ose_set_wavefield", "ose_get_step", "ose_set_step", "ose_apply_operator",
         "ose_fft3d", "ose_set_options", "ose_pass_stats", "ose_reset_stats",
         "ose_supported_extent", "ose_extent_is_supported", "ose_set_autotune", "ose_get_tuning",

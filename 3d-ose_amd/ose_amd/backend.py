@@ -99,6 +99,29 @@ class Context:
         self._check(self._L.ose_receiver_count(self._h, shot, ctypes.byref(n)))
         return n.value
 
+    def set_snapshots(self, every, callback=None, lanes=3, fmt="f32", interior=False):
+        """Stream snapshots every `every` steps (0 stops).  callback(info,
+        data) runs on the library's snapshot thread with info a dict (shot_id,
+        step, lanes, format, x0..nz, bytes) and data a bytes copy of the
+        encoded snapshot (layout of ose_set_snapshots)."""
+        if every == 0:
+            self._check(self._L.ose_set_snapshots(self._h, 0, 0, 0, 0, _lib.SNAPSHOT_FN(), None))
+            self._snap_cb = None
+            return
+
+        def trampoline(user, info, data):
+            i = info.contents
+            d = {k: getattr(i, k) for k, _ in _lib.SnapshotInfo._fields_}
+            callback(d, ctypes.string_at(data, i.bytes))
+
+        cb = _lib.SNAPSHOT_FN(trampoline)
+        self._check(self._L.ose_set_snapshots(self._h, int(every), int(lanes), _lib.SNAP_FORMATS[fmt],
+                                              int(bool(interior)), cb, None))
+        self._snap_cb = cb          # keep the C callback alive while streaming
+
+    def snapshot_drain(self):
+        self._check(self._L.ose_snapshot_drain(self._h))
+
     def copy_probe(self, nbytes=1 << 30, reps=10):
         """GB/s of a plain device copy (read + write bytes)."""
         g = c_double()

@@ -163,6 +163,41 @@ def read_params(path):
     return out
 
 
+SNAP_META = ("format", "lane", "step", "shot", "x0", "y0", "z0", "nx", "ny", "nz")
+
+
+def read_osnap(path):
+    """Decode a .osnap file: (float32 [nz][ny][nx], header dict)."""
+    meta = (ctypes.c_int * 10)()
+    rc = _L().oseh_read_osnap(str(path).encode(), None, 0, meta)
+    if rc:
+        raise OseError(rc, f"cannot read {path}")
+    m = dict(zip(SNAP_META, list(meta)))
+    out = np.empty((m["nz"], m["ny"], m["nx"]), dtype=np.float32)
+    rc = _L().oseh_read_osnap(str(path).encode(), out.ctypes.data_as(_fp), out.size, meta)
+    if rc:
+        raise OseError(rc, f"cannot read {path}")
+    return out, m
+
+
+def snapshot_lane(fmt, nl, lane, nx, ny, nz, data):
+    """One lane of a delivered snapshot (bytes) as its encoded bytes."""
+    n = _L().oseh_snapshot_lane_bytes(fmt, nx, ny, nz)
+    out = ctypes.create_string_buffer(n)
+    rc = _L().oseh_snapshot_lane(fmt, nl, lane, nx, ny, nz, data, out)
+    if rc:
+        raise OseError(rc, "oseh_snapshot_lane")
+    return out.raw
+
+
+def decode_snapshot(fmt, nx, ny, nz, lane_data):
+    out = np.empty((nz, ny, nx), dtype=np.float32)
+    rc = _L().oseh_decode_snapshot(fmt, nx, ny, nz, lane_data, out.ctypes.data_as(_fp))
+    if rc:
+        raise OseError(rc, "oseh_decode_snapshot")
+    return out
+
+
 class Simulation:
     """A preset or key=value configuration driven through the HIP backend."""
 
@@ -213,6 +248,11 @@ class Simulation:
 
     def write(self, outdir):
         self._check(self._L.oseh_sim_write(self._h, outdir.encode()))
+
+    def stream_snapshots(self, d, every):
+        """Snapshots every `every` steps into d while the run goes on (keys
+        snap_format, snap_region, snap_lanes); 0 stops."""
+        self._check(self._L.oseh_sim_stream_snapshots(self._h, str(d).encode(), int(every)))
 
     def checkpoint(self, d):
         self._check(self._L.oseh_sim_checkpoint(self._h, str(d).encode()))

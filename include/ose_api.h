@@ -106,6 +106,43 @@ int ose_get_traces(ose_ctx* ctx, int shot_id, int it0, int nsteps, float* out);
  * resumes from a checkpoint.  No counterpart in the reference.            */
 int ose_set_traces(ose_ctx* ctx, int shot_id, int it0, int nsteps, const float* in);
 
+/* ---- snapshot streaming ------------------------------------------------
+ * The reference dumps the whole padded state after every step, one
+ * synchronous copy and fwrite per lane (the commented block ose_3d.cu:258-269,
+ * write_3dfile_from_1darray_ci ose_3d_lib.cu:364-380).  Here, every `every`
+ * steps, each shot's state is encoded on the device, copied to pinned host
+ * memory on a second stream while stepping continues, and passed to `fn` on
+ * a library thread.  The box is the padded grid (interior = 0) or the
+ * interior; lanes: bit 0 Re u (P), bit 1 Im u (Q).  Encodings:
+ *   OSE_SNAP_F32   float32, exact
+ *   OSE_SNAP_BF16  bfloat16, round to nearest even
+ *   OSE_SNAP_Q16   per x-line of the box a float scale s = max|v| and int16
+ *                  q = rint(v * 32767 / s), decoded as q * (s / 32767)
+ *   OSE_SNAP_Q8    the same with int8 and 127
+ * Data layout: Q formats first hold the scales, float [lane][z][y]; then
+ * the values [lane][z][y][x], x fastest, lanes in the order P, Q.  `data`
+ * is valid during the call only; fn must not call into the context.
+ * ose_sync and ose_snapshot_drain return once every snapshot enqueued so
+ * far has been delivered.  every = 0 turns streaming off.  Snapshots are
+ * taken after ose_run's steps whose step count (after the step) is a
+ * multiple of every.                                                      */
+#define OSE_SNAP_F32 0
+#define OSE_SNAP_BF16 1
+#define OSE_SNAP_Q16 2
+#define OSE_SNAP_Q8 3
+typedef struct {
+    int shot_id;
+    int step;                /* the state after `step` steps                 */
+    int lanes, format;
+    int x0, y0, z0;          /* box origin in the padded grid                */
+    int nx, ny, nz;          /* box extents                                  */
+    size_t bytes;
+} ose_snapshot_info;
+typedef void (*ose_snapshot_fn)(void* user, const ose_snapshot_info* info, const void* data);
+int ose_set_snapshots(ose_ctx* ctx, int every, int lanes, int format, int interior,
+                      ose_snapshot_fn fn, void* user);
+int ose_snapshot_drain(ose_ctx* ctx);
+
 /* Current state u = P + iQ of a shot over the padded grid (the p2D/q2D .y
  * lanes; snapshot dump ose_3d.cu:260-270).                                 */
 int ose_get_wavefield(ose_ctx* ctx, int shot_id, float* re, float* im);

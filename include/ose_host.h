@@ -107,6 +107,24 @@ int oseh_read_segy(const char* path, float* traces, size_t cap, int* nrec, int* 
  * fastest (write_3dfile_from_1darray_ci ose_3d_lib.cu:364-380).            */
 int oseh_write_snapshot(const char* path, const float* f_zyx, int nxb, int nyb, int nzb);
 
+/* Encoded snapshots (ose_set_snapshots layouts).  Bytes of one lane of an
+ * nx x ny x nz box (0 for a bad format or box).                            */
+size_t oseh_snapshot_lane_bytes(int format, int nx, int ny, int nz);
+/* Copy lane `lane` (0-based among the nl lanes delivered) of a delivered
+ * snapshot into out (oseh_snapshot_lane_bytes bytes): its scales, if any,
+ * then its values.                                                         */
+int oseh_snapshot_lane(int format, int nl, int lane, int nx, int ny, int nz, const void* data, void* out);
+/* Decode one lane to float32 [z][y][x].                                    */
+int oseh_decode_snapshot(int format, int nx, int ny, int nz, const void* lane_data, float* out);
+/* .osnap file: 64-byte header ("OSESNAP1", then int32 format, lane (0 = P,
+ * 1 = Q), step, shot, x0, y0, z0, nx, ny, nz at byte 8.., uint64 payload
+ * bytes at 48) and one lane's encoded data.  box = {x0, y0, z0, nx, ny, nz}.
+ * Reading decodes into out (cap floats, [z][y][x]; out may be NULL to read
+ * the header only) and returns the ten header ints in meta.               */
+int oseh_write_osnap(const char* path, int format, int lane, int step, int shot, const int* box,
+                     const void* lane_data);
+int oseh_read_osnap(const char* path, float* out, size_t cap, int* meta);
+
 /* ---- simulation ------------------------------------------------------ */
 /* A simulation is built from SU-style "key=value" parameters (the reference
  * hard-codes them in main(), ose_3d.cu:75-125) or a preset:
@@ -148,6 +166,13 @@ int  oseh_sim_info(oseh_sim* sim, char* buf, size_t cap);
 int  oseh_sim_receivers(oseh_sim* sim, int shot, int* xyz, int* n);
 /* Write traces (raw + .su) and, if requested, snapshots into dir.          */
 int  oseh_sim_write(oseh_sim* sim, const char* dir);
+/* Stream snapshots every `every` steps while the simulation runs
+ * (ose_set_snapshots; 0 stops): keys snap_format=f32|bf16|q16|q8,
+ * snap_region=padded|interior, snap_lanes=pq|p|q.  Per lane and shot,
+ * float32 goes to dir/p-<it>-shot<k>.bin in the snapshot order below, the
+ * encoded formats to dir/p-<it>-shot<k>.osnap (q- for Im u).  A failed
+ * write is reported by the next oseh_sim_run or oseh_sim_write.           */
+int  oseh_sim_stream_snapshots(oseh_sim* sim, const char* dir, int every);
 /* Write the current wavefield of every shot as p-<it>-shot<k>.bin (Re u)
  * and q-<it>-shot<k>.bin (Im u) in the snapshot order (ose_3d.cu:260-270). */
 int  oseh_sim_snapshot(oseh_sim* sim, const char* dir);

@@ -1,5 +1,6 @@
 """The C++ host library (libose_host.so) against the oracle and fixtures.
 CPU only: nothing here touches the device."""
+import ctypes
 import os
 import struct
 
@@ -286,3 +287,39 @@ def test_receiver_geometry(H):
 def test_receiver_geometry_rejects_cells_outside(H, bad):
     with pytest.raises(H.OseError):
         H.Simulation("nx=20 ny=16 nz=12 boundary=4 nt=4 model=const extent=smooth " + bad).build()
+
+
+@pytest.mark.parametrize("fmt", [0, 1, 2, 3])
+def test_snapshot_codec_and_osnap_files(H, tmp_path, fmt):
+    """The C decoder and lane extraction against the numpy restatement of
+    the encodings; .osnap round trip; decode error within the format's
+    bound (bf16: 2^-8 relative; Q16/Q8: half a step of the line's scale)."""
+    import _snapcodec as sc
+    rng = np.random.default_rng(30 + fmt)
+    nz, ny, nx = 5, 4, 37
+    p = (rng.standard_normal((nz, ny, nx)) * np.logspace(-3, 3, nx)).astype(np.float32)
+    q = (rng.standard_normal((nz, ny, nx)) * 50).astype(np.float32)
+    q[1, 2] = 0.0                                   # an all-zero line: scale 0
+    data = sc.encode(fmt, [p, q])
+    lb = H._L().oseh_snapshot_lane_bytes(fmt, nx, ny, nz)
+    assert len(data) == 2 * lb
+    for lane, v in enumerate((p, q)):
+        raw = H.snapshot_lane(fmt, 2, lane, nx, ny, nz, data)
+        assert raw == sc.encode(fmt, [v])
+        dec = H.decode_snapshot(fmt, nx, ny, nz, raw)
+        s, vals = sc.encode_lane(fmt, v)
+        assert np.array_equal(dec, sc.decode_lane(fmt, (nz, ny, nx), s, vals))
+        assert (np.abs(dec - v) <= sc.bound(fmt, v)).all()
+        path = tmp_path / f"l{lane}.osnap"
+        box = (ctypes.c_int * 6)(2, 3, 4, nx, ny, nz)
+        assert H._L().oseh_write_osnap(str(path).encode(), fmt, lane, 17, 1, box, raw) == 0
+        got, meta = H.read_osnap(path)
+        assert np.array_equal(got, dec)
+        assert meta == dict(format=fmt, lane=lane, step=17, shot=1, x0=2, y0=3, z0=4, nx=nx, ny=ny, nz=nz)
+        assert len(open(path, "rb").read()) == 64 + lb
+    assert (H.decode_snapshot(fmt, nx, ny, nz, H.snapshot_lane(fmt, 2, 1, nx, ny, nz, data))[1, 2] == 0).all()
+    # a planted mistake in the restatement (scales of the wrong lane) is caught
+    if fmt >= 2:
+        bad = sc.encode(fmt, [q])[:4 * ny * nz] + sc.encode(fmt, [p])[4 * ny * nz:]
+        good = sc.encode(fmt, [p])
+        assert not np.array_equal(H.decode_snapshot(fmt, nx, ny, nz, bad), H.decode_snapshot(fmt, nx, ny, nz, good))
