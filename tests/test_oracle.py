@@ -171,3 +171,19 @@ def test_apply_L_symbol_on_plane_wave():
     u = np.exp(2j * np.pi * (-3 * x / 24 + 5 * z / 24))
     k = 2 * np.pi * np.hypot(3 / (10 * 24), 5 / (10 * 24))
     assert rel(case.oracle().apply_L(u), 2000.0 / case.R * k * u) < 1e-13
+
+
+def test_resume_from_state_is_bit_identical():
+    """The per-shot state is (P, Q, it): N steps equal N/2 steps, then N/2
+    more started from the saved (P, Q) with the wavelet from step N/2 on."""
+    t = np.load(os.path.join(GOLD, "step_tiny.npz"))
+    nx, ny, nz, bl, br, bf, bb, bu, bd, nxb, nyb, nzb = (int(x) for x in t["dims"])
+    m = orc.Model(t["v"], *t["d"], float(t["R"]), t["coef"], t["gx"], t["gy"], t["gz"])
+    n = t["s"].size
+    h = n // 2
+    args = (int(t["src"]),)
+    tr, P, Q = m.run(n, *args, t["s"], t["h"], int(t["rz"]), bl, bf, nx, ny)
+    tr1, P1, Q1 = m.run(h, *args, t["s"][:h], t["h"][:h], int(t["rz"]), bl, bf, nx, ny)
+    tr2, P2, Q2 = m.run(n - h, *args, t["s"][h:], t["h"][h:], int(t["rz"]), bl, bf, nx, ny, P=P1, Q=Q1)
+    assert np.array_equal(np.concatenate([tr1, tr2], axis=1), tr)
+    assert np.array_equal(P2, P) and np.array_equal(Q2, Q)
