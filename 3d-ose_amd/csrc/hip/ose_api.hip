@@ -338,13 +338,27 @@ int set_device(ose_ctx* c)
 }
 
 // root table exp(-2 pi i j / n), FP64 then rounded
+// Root table of one axis of extent n = N1 N2, three blocks of n entries:
+//   [0, n)    w^j, w = exp(-2 pi i / n)          (injection phase factors)
+//   [n, 2n)   w^(t k1) at t N1 + k1, t < N2     (twiddles after the N1-point
+//             DFTs: each thread's N1 values are contiguous)
+//   [2n, 3n)  w^(t n2) at t N2 + n2, t < N1     (twiddles before the inverse
+//             N1-point DFTs)
+// Angles in FP64, reduced mod n before the cos/sin.
 void roots(int n, std::vector<float2>& out)
 {
-    out.resize(n);
-    for (int j = 0; j < n; ++j) {
-        const double a = -2.0 * M_PI * (double)j / (double)n;
-        out[j] = make_float2((float)std::cos(a), (float)std::sin(a));
-    }
+    const FftEntry* e = find_entry(n);
+    const int n1 = e ? e->N1 : n, n2 = e ? e->N2 : 1;
+    out.resize(3 * (size_t)n);
+    auto w = [n](long j) {
+        const double a = -2.0 * M_PI * (double)(j % n) / (double)n;
+        return make_float2((float)std::cos(a), (float)std::sin(a));
+    };
+    for (int j = 0; j < n; ++j) out[j] = w(j);
+    for (int t = 0; t < n2; ++t)
+        for (int k1 = 0; k1 < n1; ++k1) out[n + (size_t)t * n1 + k1] = w((long)t * k1);
+    for (int t = 0; t < n1; ++t)
+        for (int k2 = 0; k2 < n2; ++k2) out[2 * (size_t)n + (size_t)t * n2 + k2] = w((long)t * k2);
 }
 
 // squared folded wavenumbers (scale_kxkz_3D, ose_3d_kernel.cu:109-123)
@@ -412,6 +426,9 @@ int launch_strided(ose_ctx* c, int kind, const StridedKern& k, const float2* in,
                    const Model* m, double bytes)
 {
     if (!k.fn || !in || !out || !tw) return c->fail(OSE_EINVAL, "strided pass launched with a missing buffer");
+    // the kernels address a thread's column with a 32-bit byte offset (col + t es) * 8, t < 32
+    if ((double)(k.T + 32.0 * (zaxis ? (double)nx * ny : (double)nx)) * sizeof(float2) >= 4294967296.0)
+        return c->fail(OSE_EUNSUPPORTED, "strided pass: plane of %d x %d cells too large", nx, ny);
     StridedArgs a{};
     a.in = in;
     a.out = out;
@@ -1181,13 +1198,13 @@ int ose_add_model(ose_ctx* c, const ose_model_desc* d, const float* v, int* mode
     roots(nx, tx);
     roots(ny, ty);
     roots(nz, tz);
-    if ((rc = dev_alloc(c, m->tw, sizeof(float2) * (nx + ny + nz)))) return rc;
+    if ((rc = dev_alloc(c, m->tw, sizeof(float2) * 3 * (nx + ny + nz)))) return rc;
     m->d_twx = (float2*)m->tw.p;
-    m->d_twy = m->d_twx + nx;
-    m->d_twz = m->d_twy + ny;
-    HIPCHK(c, hipMemcpy(m->d_twx, tx.data(), sizeof(float2) * nx, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(m->d_twy, ty.data(), sizeof(float2) * ny, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(m->d_twz, tz.data(), sizeof(float2) * nz, hipMemcpyHostToDevice));
+    m->d_twy = m->d_twx + 3 * nx;
+    m->d_twz = m->d_twy + 3 * ny;
+    HIPCHK(c, hipMemcpy(m->d_twx, tx.data(), sizeof(float2) * tx.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(m->d_twy, ty.data(), sizeof(float2) * ty.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(m->d_twz, tz.data(), sizeof(float2) * tz.size(), hipMemcpyHostToDevice));
     c->models.push_back(std::move(m));
     if (model_id) *model_id = (int)c->models.size() - 1;
     return OSE_OK;
@@ -1716,18 +1733,18 @@ int ose_fft3d(ose_ctx* c, int nx, int ny, int nz, const float* in, float* out, i
     roots(ny, ty);
     roots(nz, tz);
     DevBuf tmp;
-    if ((rc = dev_alloc(c, tmp, 2 * fb + sizeof(float2) * (nx + ny + nz) + sizeof(float) * (nx + ny + nz))))
+    if ((rc = dev_alloc(c, tmp, 2 * fb + sizeof(float2) * 3 * (nx + ny + nz) + sizeof(float) * (nx + ny + nz))))
         return rc;
     float2* A = (float2*)tmp.p;
     float2* X = A + n;
     float2* twx = X + n;
-    float2* twy = twx + nx;
-    float2* twz = twy + ny;
-    float* ones = (float*)(twz + nz);
+    float2* twy = twx + 3 * nx;
+    float2* twz = twy + 3 * ny;
+    float* ones = (float*)(twz + 3 * nz);
     std::vector<float> one(nx + ny + nz, 1.f);
-    HIPCHK(c, hipMemcpy(twx, tx.data(), sizeof(float2) * nx, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(twy, ty.data(), sizeof(float2) * ny, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(twz, tz.data(), sizeof(float2) * nz, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(twx, tx.data(), sizeof(float2) * tx.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(twy, ty.data(), sizeof(float2) * ty.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(twz, tz.data(), sizeof(float2) * tz.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(ones, one.data(), sizeof(float) * one.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(A, in, fb, hipMemcpyHostToDevice));
     const bool prof = c->profile;

@@ -51,7 +51,9 @@ struct StridedArgs {
     long es;            // element stride along the pencil
     long os;            // stride of the blockIdx.y index
     long shot_stride;   // cells per shot
-    const float2* tw;   // tw[j] = exp(-2 pi i j / N), j < N
+    const float2* tw;   // root table of the pencil axis, three blocks of N:
+                        // w^j; w^(t k1) at N + t N1 + k1; w^(t n2) at
+                        // 2N + t N2 + n2 (w = exp(-2 pi i / N))
     // symbol tables (z pass only): squared folded wavenumbers
     const float* kx2;
     const float* ky2;
@@ -113,6 +115,55 @@ __device__ __forceinline__ void col_exchange(float2* v, bool wr, bool rd, FW wro
     }
 }
 
+// Addressing of a strided pass: element (pencil row r0 + t, column col) of
+// the workgroup's T columns lives at the workgroup's base (uniform) + r0 *
+// es (uniform) + a 32-bit per-thread byte offset (col + t es) * 8, so every
+// global access takes a scalar base and one 32-bit vector offset instead of
+// a 64-bit multiply per element.  Threads past the last column load from the
+// last column (in bounds, results unused) and do not store.
+struct StridedAddr {
+    long wbase;        // element index of the workgroup's first column
+    long es;
+    unsigned boff;     // (xc + t es) * sizeof(float2)
+    int xc;            // column actually addressed
+    bool ok;           // the thread's own column exists
+
+    __device__ __forceinline__ StridedAddr(const StridedArgs& a, int T, int col, int t)
+    {
+        const int x0 = blockIdx.x * T;
+        xc = min(col, a.ncol - 1 - x0);
+        ok = col == xc;
+        es = a.es;
+        wbase = (long)blockIdx.z * a.shot_stride + (long)blockIdx.y * a.os + x0;
+        boff = ((unsigned)xc + (unsigned)t * (unsigned)a.es) * (unsigned)sizeof(float2);
+    }
+    typedef __attribute__((address_space(1))) char gchar;
+    typedef float vf2 __attribute__((ext_vector_type(2)));
+    typedef __attribute__((address_space(1))) vf2 gvf2;
+    // the wave-uniform global address of pencil row r0, pinned to scalar
+    // registers so that the compiler keeps it apart from the per-thread offset
+    __device__ __forceinline__ gchar* row(const float2* p, int r0) const
+    {
+        const unsigned long long u = reinterpret_cast<unsigned long long>(p + wbase + (long)r0 * es);
+        const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+        const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+        return (gchar*)(((unsigned long long)hi << 32) | lo);
+    }
+    // element at pencil index r0 + t
+    __device__ __forceinline__ float2 load(const float2* p, int r0) const
+    {
+        const vf2 r = *(const gvf2*)(row(p, r0) + boff);
+        return make_float2(r.x, r.y);
+    }
+    __device__ __forceinline__ void store(float2* p, int r0, float2 v) const
+    {
+        vf2 r;
+        r.x = v.x;
+        r.y = v.y;
+        *(gvf2*)(row(p, r0) + boff) = r;
+    }
+};
+
 template <int N1, int N2, int T, int SIGN, int SPLIT>
 __global__ __launch_bounds__((StridedCfg<N1, N2, T, SPLIT>::THREADS))
 void k_strided(StridedArgs a)
@@ -122,18 +173,16 @@ void k_strided(StridedArgs a)
     const int tid = threadIdx.x;
     const int col = tid % T;
     const int t = tid / T;
-    const int x = blockIdx.x * T + col;
-    const bool ok = x < a.ncol;
-    const long base = (long)blockIdx.z * a.shot_stride + (long)blockIdx.y * a.os + x;
+    const StridedAddr ad(a, T, col, t);
     float2 v[C::NT];
     if (t < N2) {
 #pragma unroll
-        for (int n1 = 0; n1 < N1; ++n1)
-            v[n1] = ok ? a.in[base + (long)(N2 * n1 + t) * a.es] : make_float2(0.f, 0.f);
+        for (int n1 = 0; n1 < N1; ++n1) v[n1] = ad.load(a.in, N2 * n1);
         rdft<N1, SIGN>(v);
+        const float2* twt = a.tw + C::N + t * N1;
 #pragma unroll
         for (int k1 = 1; k1 < N1; ++k1) {
-            const float2 w = a.tw[t * k1];
+            const float2 w = twt[k1];
             v[k1] = SIGN < 0 ? cmul(v[k1], w) : cmulc(v[k1], w);
         }
     }
@@ -141,9 +190,9 @@ void k_strided(StridedArgs a)
         v, t < N2, t < N1, [&](int k1) { return t * N1 + k1; }, [&](int n2) { return n2 * N1 + t; }, lds, col);
     if (t < N1) {
         rdft<N2, SIGN>(v);
-        if (ok) {
+        if (ad.ok) {
 #pragma unroll
-            for (int k2 = 0; k2 < N2; ++k2) a.out[base + (long)(t + N1 * k2) * a.es] = v[k2];
+            for (int k2 = 0; k2 < N2; ++k2) ad.store(a.out, N1 * k2, v[k2]);
         }
     }
 }
@@ -153,8 +202,12 @@ void k_strided(StridedArgs a)
 // In place; blockIdx.y is the (spectral) y index.  Each LDS location of the
 // second exchange is written by the thread that read it in the first, so
 // the exchanges need no barrier between them.
+// The split form asks for at least 4 waves per SIMD (at most 128 VGPRs):
+// its half-size LDS lets two 512-thread workgroups share a CU only then,
+// and the z pass needs them to cover its address translation latency.  (The
+// whole-complex form is held to one workgroup per CU by its LDS anyway.)
 template <int N1, int N2, int T, int SPLIT>
-__global__ __launch_bounds__((StridedCfg<N1, N2, T, SPLIT>::THREADS))
+__global__ __launch_bounds__((StridedCfg<N1, N2, T, SPLIT>::THREADS), (SPLIT ? 4 : 1))
 void k_zsym(StridedArgs a)
 {
     using C = StridedCfg<N1, N2, T, SPLIT>;
@@ -162,41 +215,42 @@ void k_zsym(StridedArgs a)
     const int tid = threadIdx.x;
     const int col = tid % T;
     const int t = tid / T;
-    const int x = blockIdx.x * T + col;
-    const bool ok = x < a.ncol;
-    const long base = (long)blockIdx.z * a.shot_stride + (long)blockIdx.y * a.os + x;
+    const StridedAddr ad(a, T, col, t);
     float2 v[C::NT];
     if (t < N2) {
 #pragma unroll
-        for (int n1 = 0; n1 < N1; ++n1)
-            v[n1] = ok ? a.in[base + (long)(N2 * n1 + t) * a.es] : make_float2(0.f, 0.f);
+        for (int n1 = 0; n1 < N1; ++n1) v[n1] = ad.load(a.in, N2 * n1);
         rdft<N1, -1>(v);
+        const float2* twt = a.tw + C::N + t * N1;
 #pragma unroll
-        for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], a.tw[t * k1]);
+        for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], twt[k1]);
     }
     col_exchange<T, SPLIT, N1, N2>(
         v, t < N2, t < N1, [&](int k1) { return t * N1 + k1; }, [&](int n2) { return n2 * N1 + t; }, lds, col);
     if (t < N1) {
         rdft<N2, -1>(v);
-        const float kxy = (ok ? a.kx2[x] : 0.f) + a.ky2[blockIdx.y];
+        const float kxy = a.kx2[blockIdx.x * T + ad.xc] + a.ky2[blockIdx.y];
 #pragma unroll
         for (int k2 = 0; k2 < N2; ++k2) {
-            const float s = sqrtf(kxy + a.kz2[t + N1 * k2]) * a.invN;
+            // hardware square root (1 ulp): |k| needs no correctly rounded
+            // result, and the rounding-corrected sqrtf costs ~10 instructions
+            const float s = __builtin_amdgcn_sqrtf(kxy + a.kz2[t + N1 * k2]) * a.invN;
             v[k2].x *= s;
             v[k2].y *= s;
         }
         // inverse: thread m1 = t holds X[m1 + N1 m2]
         rdft<N2, 1>(v);
+        const float2* twi = a.tw + 2 * C::N + t * N2;
 #pragma unroll
-        for (int n2 = 1; n2 < N2; ++n2) v[n2] = cmulc(v[n2], a.tw[t * n2]);
+        for (int n2 = 1; n2 < N2; ++n2) v[n2] = cmulc(v[n2], twi[n2]);
     }
     col_exchange<T, SPLIT, N2, N1>(
         v, t < N1, t < N2, [&](int n2) { return n2 * N1 + t; }, [&](int m1) { return t * N1 + m1; }, lds, col);
     if (t < N2) {
         rdft<N1, 1>(v);
-        if (ok) {
+        if (ad.ok) {
 #pragma unroll
-            for (int n1 = 0; n1 < N1; ++n1) a.out[base + (long)(N2 * n1 + t) * a.es] = v[n1];
+            for (int n1 = 0; n1 < N1; ++n1) ad.store(a.out, N2 * n1, v[n1]);
         }
     }
 }
@@ -228,7 +282,7 @@ struct XArgs {
     const float* gx;
     const float* gy;
     const float* gz;
-    const float2* tw;       // x root table (inverse transform)
+    const float2* tw;       // x root table (inverse transform; layout of StridedArgs::tw)
     const float2* twf;      // the same table for the forward transform: a
                             // separate pointer keeps the compiler from holding
                             // the inverse's twiddle loads in registers for reuse
@@ -315,8 +369,9 @@ __device__ __forceinline__ void x_inverse(const float2* __restrict__ src, bool o
         for (int m2 = 0; m2 < N2; ++m2) w[m2] = okl ? src[t + N1 * m2] : make_float2(0.f, 0.f);
         rdft<N2, 1>(w);
         if (TWB) __builtin_amdgcn_sched_barrier(0);
+        const float2* twi = tw + 2 * N1 * N2 + t * N2;
 #pragma unroll
-        for (int n2 = 1; n2 < N2; ++n2) w[n2] = cmulc(w[n2], tw[t * n2]);
+        for (int n2 = 1; n2 < N2; ++n2) w[n2] = cmulc(w[n2], twi[n2]);
     }
     line_exchange<SPLIT, N2, N1>(
         w, t < N1, res, t < N2, [&](int n2) { return n2 * PN + t; }, [&](int m1) { return t * PN + m1; }, L);
@@ -334,8 +389,9 @@ __device__ __forceinline__ void x_forward(float2* __restrict__ u, bool okl, int 
     if (t < N2) {
         rdft<N1, -1>(u);
         if (TWB) __builtin_amdgcn_sched_barrier(0);
+        const float2* twt = tw + N1 * N2 + t * N1;
 #pragma unroll
-        for (int k1 = 1; k1 < N1; ++k1) u[k1] = cmul(u[k1], tw[t * k1]);
+        for (int k1 = 1; k1 < N1; ++k1) u[k1] = cmul(u[k1], twt[k1]);
     }
     line_exchange<SPLIT, N1, N2>(
         u, t < N2, w, t < N1, [&](int k1) { return t * PN + k1; }, [&](int n2) { return n2 * PN + t; }, L);
