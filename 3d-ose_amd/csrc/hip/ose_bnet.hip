@@ -173,7 +173,7 @@ __device__ __forceinline__ int tap_offset(int tap, int oy, int oz)
 }
 
 template <int EPI>
-__global__ __launch_bounds__(256) void k_bn_conv(ConvArgs a)
+__global__ __launch_bounds__(256, 4) void k_bn_conv(ConvArgs a)
 {
     const int lane = threadIdx.x & 63;
     const int g = lane >> 4;
@@ -181,15 +181,13 @@ __global__ __launch_bounds__(256) void k_bn_conv(ConvArgs a)
     const int half = g >> 1;                 // which tap of the pair
     const int ch0 = 8 * (g & 1);             // first channel of this lane
     const int shot = blockIdx.y;
-    s16x8 wb[BN_PAIRS];
-#pragma unroll
-    for (int p = 0; p < BN_PAIRS; ++p) {
-        const int tap = 2 * p + half;
-        if (tap < BN_TAPS)
-            wb[p] = *(const s16x8*)(a.W + (tap * 16 + r16) * 16 + ch0);
-        else
-            wb[p] = s16x8{0, 0, 0, 0, 0, 0, 0, 0};
-    }
+    // the layer's weights in LDS (tap 27 zero), read per MFMA: holding all
+    // 14 pairs in registers took 56 VGPRs and capped the kernel at 3 waves
+    __shared__ __align__(16) uint16_t wl[2 * BN_PAIRS * 16 * 16];
+    for (int i = threadIdx.x; i < 2 * BN_PAIRS * 16 * 16; i += blockDim.x)
+        wl[i] = i < BN_WSZ ? a.W[i] : (uint16_t)0;
+    __syncthreads();
+    auto wb = [&](int p) { return *(const s16x8*)(wl + ((2 * p + half) * 16 + r16) * 16 + ch0); };
     const float bias = a.bias[r16];
     const long ntiles = (a.sout.count + 15) / 16;
     const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
@@ -206,27 +204,22 @@ __global__ __launch_bounds__(256) void k_bn_conv(ConvArgs a)
         }
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
         if (__all(nf.fast || !valid)) {
-            // the whole tile on constant offsets: seven pair-loads are issued
-            // together, then seven MFMAs
+            // the whole tile on constant offsets: the fourteen pair-loads are
+            // issued together, then fourteen MFMAs
             const unsigned base = valid ? nf.base : 0u;
+            s16x8 av[BN_PAIRS];
 #pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                s16x8 av[7];
-#pragma unroll
-                for (int k = 0; k < 7; ++k) {
-                    const int p = 7 * c + k;
-                    const int tap = 2 * p + half;
-                    const bool ok = valid && tap < BN_TAPS;
-                    const unsigned nb = ok ? base + (unsigned)tap_offset(tap, nf.oy, nf.oz) : 0u;
-                    av[k] = *(const s16x8*)(in + (size_t)nb * 16 + ch0);
-                    if (!ok) av[k] = s16x8{0, 0, 0, 0, 0, 0, 0, 0};
-                }
-#pragma unroll
-                for (int k = 0; k < 7; ++k)
-                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av[k]),
-                                                                  __builtin_bit_cast(bf16x8, wb[7 * c + k]), acc, 0,
-                                                                  0, 0);
+            for (int p = 0; p < BN_PAIRS; ++p) {
+                const int tap = 2 * p + half;
+                const bool ok = valid && tap < BN_TAPS;
+                const unsigned nb = ok ? base + (unsigned)tap_offset(tap, nf.oy, nf.oz) : 0u;
+                av[p] = *(const s16x8*)(in + (size_t)nb * 16 + ch0);
+                if (!ok) av[p] = s16x8{0, 0, 0, 0, 0, 0, 0, 0};
             }
+#pragma unroll
+            for (int p = 0; p < BN_PAIRS; ++p)
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av[p]),
+                                                              __builtin_bit_cast(bf16x8, wb(p)), acc, 0, 0, 0);
         } else {
 #pragma unroll
             for (int p = 0; p < BN_PAIRS; ++p) {
@@ -243,7 +236,7 @@ __global__ __launch_bounds__(256) void k_bn_conv(ConvArgs a)
                     if (nb >= 0) av = *(const s16x8*)(in + nb * 16 + ch0);
                 }
                 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av),
-                                                              __builtin_bit_cast(bf16x8, wb[p]), acc, 0, 0, 0);
+                                                              __builtin_bit_cast(bf16x8, wb(p)), acc, 0, 0, 0);
             }
         }
 #pragma unroll
