@@ -77,51 +77,6 @@ __host__ __device__ inline void shell_coords(const Shell& s, long i, int& x, int
     }
 }
 
-// Neighbour addressing of a conv from shell R_{d+1} (input) to R_d (output).
-// Away from the band edges the 27 neighbours of an output cell lie in the
-// same band of the input shell, at constant offsets from `base`:
-// nb(dz, dy, dx) = base + dz*oz + dy*oy + dx.  Bands of R_{d+1} start one
-// cell earlier and are two cells thicker than those of R_d (unless clipped
-// to the extent: `fast_ok` is then false for the whole launch).  Other cells
-// take the general shell_index path.
-struct NbFast {
-    unsigned base;
-    int oy, oz;
-    bool fast;
-};
-
-__host__ __device__ inline NbFast nb_fast(const Shell& so, const Shell& si, unsigned cell, bool fast_ok)
-{
-    NbFast r{0u, 0, 0, false};
-    const unsigned nx = (unsigned)so.nx, ny = (unsigned)so.ny;
-    if (cell < (unsigned)so.offY) {                      // z band: [bz][y][x]
-        const unsigned x = cell % nx, q = cell / nx, y = q % ny, bz = q / ny;
-        r.base = ((bz + 1u) * ny + y) * nx + x;
-        r.oy = (int)nx;
-        r.oz = (int)(ny * nx);
-        r.fast = fast_ok && x >= 1u && x + 2u <= nx && y >= 1u && y + 2u <= ny;
-    } else if (cell < (unsigned)so.offX) {               // y band: [iz][by][x]
-        const unsigned j = cell - (unsigned)so.offY;
-        const unsigned x = j % nx, q = j / nx, by = q % (unsigned)so.ty, iz = q / (unsigned)so.ty;
-        const int nzy = so.nz - so.tz;                   // rows of the y band
-        r.base = (unsigned)si.offY + ((iz - 1u) * (unsigned)si.ty + by + 1u) * nx + x;
-        r.oy = (int)nx;
-        r.oz = si.ty * (int)nx;
-        r.fast = fast_ok && x >= 1u && x + 2u <= nx && iz >= 2u && (int)iz <= nzy - 3;
-    } else {                                             // x band: [iz][iy][bx]
-        const unsigned j = cell - (unsigned)so.offX;
-        const unsigned tx = (unsigned)so.tx, nyx = (unsigned)(so.ny - so.ty);
-        const unsigned bx = j % tx, q = j / tx, iy = q % nyx, iz = q / nyx;
-        const int nzy = so.nz - so.tz;
-        const unsigned nyi = (unsigned)(si.ny - si.ty);
-        r.base = (unsigned)si.offX + ((iz - 1u) * nyi + (iy - 1u)) * (unsigned)si.tx + bx + 1u;
-        r.oy = si.tx;
-        r.oz = (int)nyi * si.tx;
-        r.fast = fast_ok && iz >= 2u && (int)iz <= nzy - 3 && iy >= 2u && (int)iy <= (int)nyx - 3;
-    }
-    return r;
-}
-
 struct BnetGeom {
     int nxb, nyb, nzb, bl, bf, bu, nx, ny, nz;
     float vmax;

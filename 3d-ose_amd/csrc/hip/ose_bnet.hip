@@ -34,6 +34,14 @@ __device__ __forceinline__ float gelu(float x)
     const float k = 0.7978845608028654f;
     return 0.5f * x * (1.f + tanhf(k * (x + 0.044715f * x * x * x)));
 }
+// the same function as x * sigmoid(2 k (x + 0.044715 x^3)) through the
+// hardware exp2 and reciprocal (~1e-6 relative; the result is rounded to bf16)
+__device__ __forceinline__ float gelu_fast(float x)
+{
+    const float k2 = -2.f * 0.7978845608028654f * 1.4426950408889634f;   // -2 k log2(e)
+    const float e = __builtin_amdgcn_exp2f(k2 * (x + 0.044715f * x * x * x));
+    return x * __builtin_amdgcn_rcpf(1.f + e);
+}
 __device__ __forceinline__ float gelu_grad(float x)
 {
     const float k = 0.7978845608028654f;
@@ -142,10 +150,33 @@ __global__ __launch_bounds__(256) void k_bn_features(FeatArgs a)
 
 // ---------------------------------------------------------------------
 // forward convolution: out[c][o] = sum_tap sum_i in[c + off(tap)][i] W[tap][o][i]
-// A = activations (16 cells x 16 in), B = W[tap] (16 in x 16 out), one
-// MFMA 16x16x16 per tap; a wave owns 16 output cells at a time.
+// BF16 implicit GEMM on MFMA 16x16x32 (two taps per instruction), FP32
+// accumulation.  The output shell's three bands are each a dense box [A][B][C]
+// (C fastest) in the compact layout, mapped to the grid by an offset:
+// (z, y, x) = (zo + A, yo + B, xo + C) modulo the extents.  A workgroup owns a
+// TA x TB x TC tile of one band: it stages the tile's (TA+2)(TB+2)(TC+2)
+// neighbour rows in LDS (each row found through the input shell's index,
+// zero where the neighbour is not in the shell) and reads every MFMA's A
+// operand from there: 81 B of global reads per output cell instead of
+// 27 x 32 B of gathers.
 // ---------------------------------------------------------------------
 enum { EPI_HIDDEN = 0, EPI_OUT = 1 };
+
+// one band of the output shell
+struct ConvBand {
+    long off_o;                 // compact index of the band's first cell
+    int nA, nB, nC;             // box extents
+    int zo, yo, xo;             // grid coordinates of box cell (0, 0, 0)
+    int tA, tB, tC;             // tiles along A, B, C
+    long tile0;                 // first tile of this band
+};
+
+constexpr int CT_A = 4, CT_B = 4, CT_C = 16;
+constexpr int CH_A = CT_A + 2, CH_B = CT_B + 2, CH_C = CT_C + 2;
+constexpr int CH_ROWS = CH_A * CH_B * CH_C;
+// halo row stride in bf16 values: 48 bytes, so the 16-byte reads of eight
+// consecutive rows (one LDS cycle of a 128-bit read) hit disjoint banks
+constexpr int CH_LD = 24;
 
 struct ConvArgs {
     const uint16_t* in;
@@ -157,98 +188,133 @@ struct ConvArgs {
     long h_shot;
     float* yout;            // EPI_HIDDEN: y fp32 [n_out][16] (optional); EPI_OUT: [n_out][2]
     long y_shot;
+    ConvBand band[3];
 };
 
 // Two taps per MFMA (16x16x32, K = 2 taps x 16 channels): lane group g
 // (lanes 16g..16g+15) supplies channels 8(g%2)..8(g%2)+7 of tap 2p + g/2 for
-// the A operand (one 16-byte load) and the matching weights for B.  Tap 27
+// the A operand (one 16-byte read) and the matching weights for B.  Tap 27
 // does not exist: its weights are zero.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef short s16x8 __attribute__((ext_vector_type(8)));
 constexpr int BN_PAIRS = (BN_TAPS + 1) / 2;
 
-__device__ __forceinline__ int tap_offset(int tap, int oy, int oz)
+// v mod n (the division only for the rare v outside [0, n))
+__device__ __forceinline__ int modn(int v, int n)
 {
-    return (tap / 9 - 1) * oz + ((tap / 3) % 3 - 1) * oy + (tap % 3 - 1);
+    if (v < 0 || v >= n) {
+        v %= n;
+        if (v < 0) v += n;
+    }
+    return v;
 }
 
 template <int EPI>
-__global__ __launch_bounds__(256, 4) void k_bn_conv(ConvArgs a)
+__global__ __launch_bounds__(256) void k_bn_conv(ConvArgs a)
 {
+    __shared__ __align__(16) uint16_t halo[CH_ROWS * CH_LD];
     const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
     const int g = lane >> 4;
     const int r16 = lane & 15;
     const int half = g >> 1;                 // which tap of the pair
     const int ch0 = 8 * (g & 1);             // first channel of this lane
     const int shot = blockIdx.y;
-    // the layer's weights in LDS (tap 27 zero), read per MFMA: holding all
-    // 14 pairs in registers took 56 VGPRs and capped the kernel at 3 waves
-    __shared__ __align__(16) uint16_t wl[2 * BN_PAIRS * 16 * 16];
-    for (int i = threadIdx.x; i < 2 * BN_PAIRS * 16 * 16; i += blockDim.x)
-        wl[i] = i < BN_WSZ ? a.W[i] : (uint16_t)0;
-    __syncthreads();
-    auto wb = [&](int p) { return *(const s16x8*)(wl + ((2 * p + half) * 16 + r16) * 16 + ch0); };
-    const float bias = a.bias[r16];
-    const long ntiles = (a.sout.count + 15) / 16;
-    const long nwaves = (long)gridDim.x * (blockDim.x >> 6);
     const uint16_t* in = a.in + (long)shot * a.in_shot;
-    const bool fast_ok = a.sin.tz == a.sout.tz + 2 && a.sin.ty == a.sout.ty + 2 && a.sin.tx == a.sout.tx + 2;
-    for (long tile = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); tile < ntiles; tile += nwaves) {
-        const long cell = tile * 16 + r16;
-        const bool valid = cell < a.sout.count;
-        NbFast nf{0u, 0, 0, false};
-        int x = 0, y = 0, z = 0;
-        if (valid) {
-            nf = nb_fast(a.sout, a.sin, (unsigned)cell, fast_ok);
-            if (!nf.fast) shell_coords(a.sout, cell, x, y, z);
+
+    // the workgroup's tile: band, then A, B, C tile coordinates
+    const long t = blockIdx.x;
+    const int bi = t >= a.band[2].tile0 ? 2 : (t >= a.band[1].tile0 ? 1 : 0);
+    const ConvBand& B = a.band[bi];
+    const long tl = t - B.tile0;
+    const int tc = (int)(tl % B.tC);
+    const long q = tl / B.tC;
+    const int tb = (int)(q % B.tB);
+    const int ta = (int)(q / B.tB);
+    const int a0 = ta * CT_A, b0 = tb * CT_B, c0 = tc * CT_C;
+    const int a1 = min(a0 + CT_A, B.nA), b1 = min(b0 + CT_B, B.nB), c1 = min(c0 + CT_C, B.nC);
+
+    // stage the neighbour rows: halo (ia, ib, ic) is the input shell's row at
+    // grid cell (zo + a0 + ia - 1, yo + b0 + ib - 1, xo + c0 + ic - 1)
+    for (int row = threadIdx.x; row < CH_ROWS; row += blockDim.x) {
+        const int ic = row % CH_C, ib = (row / CH_C) % CH_B, ia = row / (CH_C * CH_B);
+        const int z = modn(B.zo + a0 + ia - 1, a.sin.nz);
+        const int y = modn(B.yo + b0 + ib - 1, a.sin.ny);
+        const int x = modn(B.xo + c0 + ic - 1, a.sin.nx);
+        const long nb = shell_index(a.sin, x, y, z);
+        s16x8 v0 = {0, 0, 0, 0, 0, 0, 0, 0}, v1 = v0;
+        if (nb >= 0) {
+            v0 = *(const s16x8*)(in + nb * 16);
+            v1 = *(const s16x8*)(in + nb * 16 + 8);
         }
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        if (__all(nf.fast || !valid)) {
-            // the whole tile on constant offsets: the fourteen pair-loads are
-            // issued together, then fourteen MFMAs
-            const unsigned base = valid ? nf.base : 0u;
-            s16x8 av[BN_PAIRS];
+        *(s16x8*)(halo + row * CH_LD) = v0;
+        *(s16x8*)(halo + row * CH_LD + 8) = v1;
+    }
+    s16x8 wb[BN_PAIRS];
 #pragma unroll
-            for (int p = 0; p < BN_PAIRS; ++p) {
-                const int tap = 2 * p + half;
-                const bool ok = valid && tap < BN_TAPS;
-                const unsigned nb = ok ? base + (unsigned)tap_offset(tap, nf.oy, nf.oz) : 0u;
-                av[p] = *(const s16x8*)(in + (size_t)nb * 16 + ch0);
-                if (!ok) av[p] = s16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    for (int p = 0; p < BN_PAIRS; ++p) {
+        const int tap = 2 * p + half;
+        wb[p] = tap < BN_TAPS ? *(const s16x8*)(a.W + (tap * 16 + r16) * 16 + ch0) : s16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    }
+    const float bias = a.bias[r16];
+    __syncthreads();
+
+    // wave w owns the rows (A, B) = (a0 + w, b0 .. b0 + 3), one MFMA tile of
+    // 16 C-consecutive cells each
+    f32x4 acc[CT_B];
+#pragma unroll
+    for (int k = 0; k < CT_B; ++k) acc[k] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int p = 0; p < BN_PAIRS; ++p) {
+        const int tap = min(2 * p + half, BN_TAPS - 1);      // tap 27: zero weights
+        const int dz = tap / 9, dy = (tap / 3) % 3, dx = tap % 3;
+#pragma unroll
+        for (int k = 0; k < CT_B; ++k) {
+            const int row = ((wave + dz) * CH_B + (k + dy)) * CH_C + (r16 + dx);
+            const s16x8 av = *(const s16x8*)(halo + row * CH_LD + ch0);
+            acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av),
+                                                              __builtin_bit_cast(bf16x8, wb[p]), acc[k], 0, 0, 0);
+        }
+    }
+    // bias, activation, store: lane (r16, g) holds cells c0 + 4g + j, channel r16
+    const int A = a0 + wave;
+    if (EPI == EPI_HIDDEN) {
+        // through LDS to cell-major rows, so that each lane stores 16 bytes
+        __shared__ __align__(16) uint16_t outl[4][CT_B][16][16];
+#pragma unroll
+        for (int k = 0; k < CT_B; ++k)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float yv = acc[k][j] + bias;
+                outl[wave][k][4 * g + j][r16] = f2bf(gelu_fast(yv));
+                if (a.yout && A < a1 && b0 + k < b1 && 4 * g + j < c1 - c0)
+                    a.yout[(long)shot * a.y_shot + (B.off_o + ((long)A * B.nB + b0 + k) * B.nC + c0 + 4 * g + j) * 16 +
+                           r16] = yv;
             }
+        __syncthreads();
+        if (A >= a1) return;
+        // lane: tile kk = 2 i + lane / 32, cell (lane % 32) / 2, half lane % 2
 #pragma unroll
-            for (int p = 0; p < BN_PAIRS; ++p)
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av[p]),
-                                                              __builtin_bit_cast(bf16x8, wb(p)), acc, 0, 0, 0);
-        } else {
-#pragma unroll
-            for (int p = 0; p < BN_PAIRS; ++p) {
-                const int tap = 2 * p + half;
-                s16x8 av = {0, 0, 0, 0, 0, 0, 0, 0};
-                if (valid && tap < BN_TAPS) {
-                    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-                    long nb;
-                    if (nf.fast)
-                        nb = (long)(nf.base + (unsigned)(dz * nf.oz + dy * nf.oy + dx));
-                    else
-                        nb = shell_index(a.sin, wrapc(x + dx, a.sin.nx), wrapc(y + dy, a.sin.ny),
-                                         wrapc(z + dz, a.sin.nz));
-                    if (nb >= 0) av = *(const s16x8*)(in + nb * 16 + ch0);
-                }
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av),
-                                                              __builtin_bit_cast(bf16x8, wb(p)), acc, 0, 0, 0);
+        for (int i = 0; i < CT_B / 2; ++i) {
+            const int kk = 2 * i + (lane >> 5), cell = (lane & 31) >> 1, hh = lane & 1;
+            const int Bc = b0 + kk;
+            const s16x8 v = *(const s16x8*)&outl[wave][kk][cell][8 * hh];
+            if (Bc < b1 && cell < c1 - c0) {
+                const long oc = B.off_o + ((long)A * B.nB + Bc) * B.nC + c0 + cell;
+                *(s16x8*)(a.hout + (long)shot * a.h_shot + oc * 16 + 8 * hh) = v;
             }
         }
+    } else {
+        if (A >= a1) return;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const long oc = tile * 16 + 4 * g + j;
-            if (oc >= a.sout.count) continue;
-            const float y = acc[j] + bias;
-            if (EPI == EPI_HIDDEN) {
-                a.hout[(long)shot * a.h_shot + oc * 16 + r16] = f2bf(gelu(y));
-                if (a.yout) a.yout[(long)shot * a.y_shot + oc * 16 + r16] = y;
-            } else if (r16 < 2) {
-                a.yout[(long)shot * a.y_shot + oc * 2 + r16] = y;
+        for (int k = 0; k < CT_B; ++k) {
+            const int Bc = b0 + k;
+            if (Bc >= b1) continue;
+            const long idx0 = B.off_o + ((long)A * B.nB + Bc) * B.nC + c0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int c = 4 * g + j;
+                if (c < c1 - c0 && r16 < 2) a.yout[(long)shot * a.y_shot + (idx0 + c) * 2 + r16] = acc[k][j] + bias;
             }
         }
     }
@@ -557,10 +623,38 @@ static int conv_layer(hipStream_t st, int epi, const uint16_t* in, long in_shot,
                       const uint16_t* W, const float* bias, uint16_t* hout, long h_shot, float* yout, long y_shot,
                       int nshot)
 {
-    ConvArgs a{in, in_shot, sin, sout, W, bias, hout, h_shot, yout, y_shot};
+    ConvArgs a{};
+    a.in = in;
+    a.in_shot = in_shot;
+    a.sin = sin;
+    a.sout = sout;
+    a.W = W;
+    a.bias = bias;
+    a.hout = hout;
+    a.h_shot = h_shot;
+    a.yout = yout;
+    a.y_shot = y_shot;
+    const int nx = sout.nx, ny = sout.ny, nz = sout.nz;
+    // z band [bz][y][x], y band [iz][by][x], x band [iz][iy][bx] (make_shell)
+    const int dims[3][3] = {{sout.tz, ny, nx}, {nz - sout.tz, sout.ty, nx}, {nz - sout.tz, ny - sout.ty, sout.tx}};
+    const int org[3][3] = {{sout.bz0, 0, 0}, {sout.bz0 + sout.tz, sout.by0, 0},
+                           {sout.bz0 + sout.tz, sout.by0 + sout.ty, sout.bx0}};
+    const long off_o[3] = {0, sout.offY, sout.offX};
+    long tiles = 0;
+    for (int b = 0; b < 3; ++b) {
+        ConvBand& c = a.band[b];
+        c.off_o = off_o[b];
+        c.nA = dims[b][0], c.nB = dims[b][1], c.nC = dims[b][2];
+        c.zo = org[b][0], c.yo = org[b][1], c.xo = org[b][2];
+        c.tA = (c.nA + CT_A - 1) / CT_A, c.tB = (c.nB + CT_B - 1) / CT_B, c.tC = (c.nC + CT_C - 1) / CT_C;
+        if (c.nA <= 0 || c.nB <= 0 || c.nC <= 0) c.tA = c.tB = c.tC = 0;
+        c.tile0 = tiles;
+        tiles += (long)c.tA * c.tB * c.tC;
+    }
+    if (tiles == 0) return OSE_OK;
+    if (tiles > 0x7fffffffL) return OSE_EINVAL;
     void* args[] = {&a};
-    const long tiles = (sout.count + 15) / 16;
-    dim3 grid(blocks_for(tiles, 4, 2048), nshot);
+    dim3 grid((unsigned)tiles, nshot);
     const void* fn = epi == EPI_HIDDEN ? (const void*)&k_bn_conv<EPI_HIDDEN> : (const void*)&k_bn_conv<EPI_OUT>;
     LCHK(hipLaunchKernel(fn, grid, dim3(256), args, 0, st));
     return OSE_OK;

@@ -37,6 +37,32 @@ def test_bnet_forward_matches_fp64_reference():
     assert rel(got, ref) < 2e-2
 
 
+def test_bnet_forward_tiled_path_matches_fp64_reference():
+    """A grid with an 8-cell pad on every side and a 64-cell x extent: the
+    conv's LDS-staged tiles (full and partial, in all three bands) run next
+    to the gathered edge tiles."""
+    import ose_amd
+    rng = np.random.default_rng(5)
+    shape = (40, 44, 64)                 # nzb, nyb, nxb
+    origin = (8, 8, 8)
+    interior = (48, 28, 24)
+    geo = B.Geometry(shape, origin, interior)
+    v = rng.uniform(1500.0, 4000.0, shape).astype(np.float32)
+    u = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    p = (0.15 * rng.standard_normal(B.NPAR)).astype(np.float32)
+    print("bnet forward 64x44x40", flush=True)
+    net = ose_amd.BoundaryNet(v, origin, interior)
+    net.set_params(p)
+    got = net.forward(u)
+    ref, _ = B.forward(geo, p, u.astype(np.complex128), v.astype(np.float64), float(v.max()))
+    net.close()
+    assert np.all(got[~geo.pad] == 0)
+    assert rel(got, ref) < 2e-2
+    # every pad cell, not only the norm: no cell may be far off
+    pad = geo.pad
+    assert np.abs(got[pad] - ref[pad]).max() < 0.1 * np.abs(ref[pad]).max()
+
+
 def test_bnet_gradient_matches_fp64_reference():
     import ose_amd
     geo, shape, origin, interior, v, u, t, p = _geometry_case(1)
