@@ -58,6 +58,31 @@ __global__ void k_inject(InjectArgs a)
     }
 }
 
+__global__ void k_inject_prof(ProfArgs a)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int shot = blockIdx.z;
+    const ProfDesc d = a.desc[shot];
+    const int iy = blockIdx.y % a.wy_max, iz = blockIdx.y / a.wy_max;
+    if (k >= a.nx || iy >= d.wy || iz >= d.wz) return;
+    const int itv = *a.it;
+    if (itv >= a.nt) return;
+    const float* pv = a.pv + (long)shot * 3 * PROF_W;
+    const float w = pv[PROF_W + iy] * pv[2 * PROF_W + iz];
+    const float ls = a.s[(long)shot * a.nt + itv] * w;
+    const float lh = a.h[(long)shot * a.nt + itv] * w;
+    const long line = (long)shot * a.shot_stride + ((long)(d.z0 + iz) * a.ny + d.y0 + iy) * a.nx;
+    const float2 e = a.PX[(long)shot * a.nx + k];
+    float2* p = a.X0 + line + k;
+    const float2 o = *p;
+    *p = make_float2(o.x + ls * e.x - lh * e.y, o.y + ls * e.y + lh * e.x);
+    if (k < d.wx) {
+        float2* q = a.W0 + line + d.x0 + k;
+        const float2 u = *q;
+        *q = make_float2(u.x + ls * pv[k], u.y + lh * pv[k]);
+    }
+}
+
 __global__ void k_advance(int* it)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) *it = *it + 1;
@@ -204,6 +229,11 @@ struct Model {
     float *d_s = nullptr, *d_h = nullptr;
     int rec_ny = 0, rec_nx = 0;    // plane receivers (0 x 0 when every shot has a list)
     // listed receivers of the batch (ose_set_receivers)
+    DevBuf pbuf;                   // extended sources: descriptors, PX, profiles
+    ProfDesc* d_pdesc = nullptr;
+    float2* d_PX = nullptr;
+    float* d_pv = nullptr;
+    int prof_wy = 0, prof_wz = 0;  // 0: point sources only (k_inject)
     DevBuf rdesc, rtr;
     RecvDesc* d_rdesc = nullptr;
     float* d_rtr = nullptr;
@@ -230,6 +260,9 @@ struct Shot {
     int nt = 0;
     std::vector<long> rec;         // listed receivers (padded cells); empty: the plane
     long rbase = 0;                // offset of this shot's samples in the model's list traces
+    // extended source (ose_set_source_profile); wx == 0: the point (sx, sy, sz)
+    ProfDesc prof{0, 0, 0, 0, 0, 0};
+    std::vector<float> px, py, pz;
 };
 
 struct Pending {
@@ -516,7 +549,25 @@ int enqueue_model_step(ose_ctx* c, Model& m, int* d_it)
     const long cells = m.ncell * nshot;
     int rc;
     // source injection into X0 (x-spectral) and W_0 (physical)
-    {
+    if (m.prof_wy > 0) {
+        ProfArgs a{};
+        a.X0 = m.X;
+        a.W0 = m.W[0];
+        a.shot_stride = m.ncell;
+        a.nx = nx;
+        a.ny = ny;
+        a.desc = m.d_pdesc;
+        a.PX = m.d_PX;
+        a.pv = m.d_pv;
+        a.s = m.d_s;
+        a.h = m.d_h;
+        a.nt = m.nt_cap;
+        a.it = d_it;
+        a.wy_max = m.prof_wy;
+        if ((rc = launch(c, PK_INJ, 16.0 * nx * m.prof_wy * m.prof_wz * nshot, (const void*)&k_inject_prof,
+                         dim3((nx + 255) / 256, m.prof_wy * m.prof_wz, nshot), dim3(256), 0, &a)))
+            return rc;
+    } else {
         InjectArgs a{};
         a.X0 = m.X;
         a.W0 = m.W[0];
@@ -799,6 +850,54 @@ int finalize(ose_ctx* c)
         if (wb) {
             HIPCHK(c, hipMemcpy(m.d_s, s.data(), wb, hipMemcpyHostToDevice));
             HIPCHK(c, hipMemcpy(m.d_h, h.data(), wb, hipMemcpyHostToDevice));
+        }
+        // extended sources: every shot of a model with one goes through the
+        // profile kernel (a point becomes the unit profile)
+        bool anyprof = false;
+        for (int id : m.shots) anyprof |= c->shots[id].prof.wx > 0;
+        m.prof_wy = m.prof_wz = 0;
+        if (anyprof) {
+            const int nxb = m.d.nxb;
+            std::vector<ProfDesc> desc(nshot);
+            std::vector<float2> PX((size_t)nshot * nxb);
+            std::vector<float> pv((size_t)nshot * 3 * PROF_W, 0.f);
+            for (int k = 0; k < nshot; ++k) {
+                const Shot& sh = c->shots[m.shots[k]];
+                ProfDesc d = sh.prof;
+                float* v = pv.data() + (size_t)k * 3 * PROF_W;
+                if (d.wx == 0) {
+                    d = ProfDesc{sh.sx, sh.sy, sh.sz, 1, 1, 1};
+                    v[0] = v[PROF_W] = v[2 * PROF_W] = 1.f;
+                } else {
+                    std::copy(sh.px.begin(), sh.px.end(), v);
+                    std::copy(sh.py.begin(), sh.py.end(), v + PROF_W);
+                    std::copy(sh.pz.begin(), sh.pz.end(), v + 2 * PROF_W);
+                }
+                desc[k] = d;
+                m.prof_wy = std::max(m.prof_wy, d.wy);
+                m.prof_wz = std::max(m.prof_wz, d.wz);
+                // PX[kx] = sum_j px[j] exp(-2 pi i kx (x0 + j) / nx), in FP64
+                for (int kx = 0; kx < nxb; ++kx) {
+                    double re = 0.0, im = 0.0;
+                    for (int j = 0; j < d.wx; ++j) {
+                        const long ph = ((long)kx * (d.x0 + j)) % nxb;
+                        const double a = -2.0 * M_PI * (double)ph / (double)nxb;
+                        re += (double)v[j] * std::cos(a);
+                        im += (double)v[j] * std::sin(a);
+                    }
+                    PX[(size_t)k * nxb + kx] = make_float2((float)re, (float)im);
+                }
+            }
+            const size_t db = sizeof(ProfDesc) * nshot, xb = sizeof(float2) * PX.size(),
+                         vb = sizeof(float) * pv.size();
+            if ((rc = dev_alloc(c, m.pbuf, db + xb + vb + 256))) return rc;
+            char* pb = (char*)m.pbuf.p;
+            m.d_PX = (float2*)pb;
+            m.d_pv = (float*)(pb + xb);
+            m.d_pdesc = (ProfDesc*)(pb + xb + vb);
+            HIPCHK(c, hipMemcpy(m.d_PX, PX.data(), xb, hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(m.d_pv, pv.data(), vb, hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(m.d_pdesc, desc.data(), db, hipMemcpyHostToDevice));
         }
         if (m.learned && (rc = alloc_bnet_bufs(c, m, nshot, false))) return rc;
     }
@@ -1256,6 +1355,38 @@ int ose_set_receivers(ose_ctx* c, int shot_id, const int* xyz, int n)
         rec[(size_t)r] = ((long)z * m.d.nyb + y) * m.d.nxb + x;
     }
     sh->rec.swap(rec);
+    return OSE_OK;
+}
+
+int ose_set_source_profile(ose_ctx* c, int shot_id, const int* box, const float* px, const float* py,
+                           const float* pz)
+{
+    if (!c) return OSE_EINVAL;
+    if (c->finalized) return c->fail(OSE_ESTATE, "ose_set_source_profile: context already running");
+    Shot* sh = get_shot(c, shot_id);
+    if (!sh) return c->fail(OSE_EINVAL, "ose_set_source_profile: bad shot");
+    if (!box) {
+        sh->prof = ProfDesc{0, 0, 0, 0, 0, 0};
+        sh->px.clear(), sh->py.clear(), sh->pz.clear();
+        return OSE_OK;
+    }
+    const Model& m = *c->models[sh->model];
+    const int x0 = box[0], y0 = box[1], z0 = box[2], wx = box[3], wy = box[4], wz = box[5];
+    const int n[3] = {m.d.nxb, m.d.nyb, m.d.nzb}, o[3] = {x0, y0, z0}, w[3] = {wx, wy, wz};
+    for (int a = 0; a < 3; ++a)
+        if (w[a] < 1 || w[a] > PROF_W || o[a] < 0 || o[a] + w[a] > n[a])
+            return c->fail(OSE_EINVAL, "ose_set_source_profile: support %d+%d on axis %d outside [0, %d) or wider than %d",
+                           o[a], w[a], a, n[a], PROF_W);
+    if (!px || !py || !pz) return c->fail(OSE_EINVAL, "ose_set_source_profile: null profile");
+    for (int a = 0; a < 3; ++a) {
+        const float* p = a == 0 ? px : a == 1 ? py : pz;
+        for (int j = 0; j < w[a]; ++j)
+            if (!std::isfinite(p[j])) return c->fail(OSE_ENUMERIC, "ose_set_source_profile: non-finite value");
+    }
+    sh->prof = ProfDesc{x0, y0, z0, wx, wy, wz};
+    sh->px.assign(px, px + wx);
+    sh->py.assign(py, py + wy);
+    sh->pz.assign(pz, pz + wz);
     return OSE_OK;
 }
 

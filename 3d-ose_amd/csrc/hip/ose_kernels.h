@@ -517,7 +517,31 @@ struct InjectArgs {
     const float2* tw;       // x root table
 };
 
+// Extended (separable) source profiles: shot k adds (s + i h) px[x] py[y]
+// pz[z] over the box [x0, x0+wx) x [y0, y0+wy) x [z0, z0+wz) to W_0 and, on
+// the box's (y, z) lines, (s + i h) py pz PX[kx] to X_0, PX the x transform
+// of px.  A point source is the unit profile.
+constexpr int PROF_W = 64;                // widest support along one axis
+struct ProfDesc {
+    int x0, y0, z0, wx, wy, wz;
+};
+struct ProfArgs {
+    float2* X0;
+    float2* W0;
+    long shot_stride;
+    int nx, ny;
+    const ProfDesc* desc;   // [shot]
+    const float2* PX;       // [shot][nx]
+    const float* pv;        // [shot][3][PROF_W]: px | py | pz
+    const float* s;         // [shot][nt]
+    const float* h;
+    int nt;
+    const int* it;
+    int wy_max;             // lines per shot in the launch: wy_max * wz_max
+};
+
 __global__ void k_inject(InjectArgs a);
+__global__ void k_inject_prof(ProfArgs a);
 __global__ void k_advance(int* it);
 __global__ void k_scale_vr(float2* f, const float* vr, long n, long shot_stride, float s);
 

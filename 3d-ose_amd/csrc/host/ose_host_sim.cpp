@@ -394,7 +394,14 @@ int oseh_sim_build(oseh_sim* s)
     std::vector<double> w(nt), hh(nt);
     oseh_ricker(w.data(), nt, s->getd("fm", 25.0), s->getd("dt", 1e-3), s->getd("amp", 100.0),
                 s->geti("delay", 40));
-    oseh_hilbert(w.data(), hh.data(), nt, s->get("hilbert", "unnorm") == "norm" ? 1 : 0);
+    // hilbert=unnorm (the reference: nt H[s]), norm (H[s]) or analytic (-H[s]:
+    // with the exp(-i L dt) step, s - i H[s] is the analytic signal that radiates)
+    const std::string hm = s->get("hilbert", "unnorm");
+    if (hm != "unnorm" && hm != "norm" && hm != "analytic")
+        return s->fail(OSE_EINVAL, "hilbert must be unnorm, norm or analytic");
+    oseh_hilbert(w.data(), hh.data(), nt, hm == "unnorm" ? 0 : 1);
+    if (hm == "analytic")
+        for (double& v : hh) v = -v;
     s->s.assign(w.begin(), w.end());
     s->h.assign(hh.begin(), hh.end());
     s->models.clear();
@@ -413,6 +420,24 @@ int oseh_sim_build(oseh_sim* s)
         }
     }
     return build_receivers(s);
+}
+
+int oseh_gaussian_profile(double sigma, int center, int n, int* start, float* out, int cap)
+{
+    if (!start || !out || center < 0 || center >= n || sigma < 0.0) return -1;
+    const int r = sigma > 0.0 ? (int)std::ceil(4.0 * sigma) : 0;
+    const int a = std::max(0, center - r), b = std::min(n - 1, center + r);
+    if (b - a + 1 > cap) return -1;
+    double sum = 0.0;
+    std::vector<double> g(b - a + 1);
+    for (int j = a; j <= b; ++j) {
+        const double t = sigma > 0.0 ? (j - center) / sigma : 0.0;
+        g[j - a] = std::exp(-0.5 * t * t);
+        sum += g[j - a];
+    }
+    for (int j = a; j <= b; ++j) out[j - a] = (float)(g[j - a] / sum);
+    *start = a;
+    return b - a + 1;
 }
 
 int oseh_sim_setup(oseh_sim* s)
@@ -458,6 +483,24 @@ int oseh_sim_setup(oseh_sim* s)
         if ((rc = ose_add_shot(s->ctx, m.id, r.src[0], r.src[1], r.src[2], s->s.data(), s->h.data(), nt, &sid)))
             return s->fail(rc, std::string("ose_add_shot: ") + ose_last_error(s->ctx));
         m.shot_ids.push_back(sid);
+        // extended source: src_sigma=<cells> or auto (spectrum e^{-k^2 sigma^2 / 2}
+        // below 1e-3 at the temporal Nyquist pi / (v_src dt): sigma = 3.72 v dt / pi)
+        const std::string ss = s->get("src_sigma", "0");
+        if (ss != "0" && ss != "0.0") {
+            const double d3[3] = {s->getd("dx", 10), s->getd("dy", 10), s->getd("dz", 10)};
+            const int n3[3] = {m.nxb, m.nyb, m.nzb};
+            const double vs = m.v[((size_t)r.src[2] * m.nyb + r.src[1]) * m.nxb + r.src[0]];
+            int box[6];
+            float prof[3][64];
+            for (int a = 0; a < 3; ++a) {
+                const double sig = ss == "auto" ? 3.72 * vs * s->getd("dt", 1e-3) / (M_PI * d3[a]) : std::atof(ss.c_str());
+                const int w = oseh_gaussian_profile(sig, r.src[a], n3[a], &box[a], prof[a], 64);
+                if (w < 1) return s->fail(OSE_EINVAL, "src_sigma " + ss + ": source profile wider than 64 cells");
+                box[3 + a] = w;
+            }
+            if ((rc = ose_set_source_profile(s->ctx, sid, box, prof[0], prof[1], prof[2])))
+                return s->fail(rc, std::string("ose_set_source_profile: ") + ose_last_error(s->ctx));
+        }
         if (!r.rec.empty()) {
             std::vector<int> xyz(r.rec.size());
             for (size_t i = 0; i < r.rec.size(); i += 3) {

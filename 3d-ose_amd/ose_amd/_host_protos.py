@@ -53,6 +53,7 @@ PROTOS = {
     "oseh_sim_model_velocity": ([vp, c_int, fp], c_int),
     "oseh_sim_model_params": ([vp, c_int, dp, ip, dp, fp, fp, fp, ip], c_int),
     "oseh_sim_nshots": ([vp], c_int),
+    "oseh_gaussian_profile": ([c_double, c_int, c_int, ip, fp, c_int], c_int),
     "oseh_sim_receivers": ([vp, c_int, ip, ip], c_int),
     "oseh_sim_shot": ([vp, c_int, ip, ip, ip, fp, fp], c_int),
 }

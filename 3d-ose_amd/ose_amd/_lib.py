@@ -77,6 +77,7 @@ def hip():
         _proto(L, "ose_set_traces", [vp, c_int, c_int, c_int, fp])
         _proto(L, "ose_set_receivers", [vp, c_int, ip, c_int])
         _proto(L, "ose_receiver_count", [vp, c_int, ip])
+        _proto(L, "ose_set_source_profile", [vp, c_int, ip, fp, fp, fp])
         _proto(L, "ose_set_snapshots", [vp, c_int, c_int, c_int, c_int, SNAPSHOT_FN, vp])
         _proto(L, "ose_snapshot_drain", [vp])
         _proto(L, "ose_get_wavefield", [vp, c_int, fp, fp])
@@ -130,7 +131,7 @@ def hip_symbols():
     return [
         "ose_create", "ose_destroy", "ose_last_error", "ose_add_model", "ose_add_shot",
         "ose_run", "ose_sync", "ose_timed_run", "ose_get_traces", "ose_set_traces", "ose_set_receivers",
-        "ose_receiver_count", "ose_set_snapshots", "ose_snapshot_drain", "ose_get_wavefield",
+        "ose_receiver_count", "ose_set_source_profile", "ose_set_snapshots", "ose_snapshot_drain", "ose_get_wavefield",
         "ose_set_wavefield", "ose_get_step", "ose_set_step", "ose_apply_operator",
         "ose_fft3d", "ose_set_options", "ose_pass_stats", "ose_reset_stats",
         "ose_supported_extent", "ose_extent_is_supported", "ose_set_autotune", "ose_get_tuning",

@@ -94,6 +94,16 @@ class Context:
         a = np.ascontiguousarray(np.asarray(xyz, dtype=np.int32).reshape(-1, 3))
         self._check(self._L.ose_set_receivers(self._h, shot, a.ctypes.data_as(ctypes.POINTER(c_int)), len(a)))
 
+    def set_source_profile(self, shot, origin=None, px=None, py=None, pz=None):
+        """Extended source: (s + i h) px py pz over the padded box starting at
+        origin = (x0, y0, z0); origin None restores the point source."""
+        if origin is None:
+            self._check(self._L.ose_set_source_profile(self._h, shot, None, None, None, None))
+            return
+        px, py, pz = _f32(px), _f32(py), _f32(pz)
+        box = (c_int * 6)(int(origin[0]), int(origin[1]), int(origin[2]), px.size, py.size, pz.size)
+        self._check(self._L.ose_set_source_profile(self._h, shot, box, _ptr(px), _ptr(py), _ptr(pz)))
+
     def receiver_count(self, shot):
         n = c_int()
         self._check(self._L.ose_receiver_count(self._h, shot, ctypes.byref(n)))

@@ -163,6 +163,17 @@ def read_params(path):
     return out
 
 
+def gaussian_profile(sigma, center, n, cap=64):
+    """(start, values): normalised Gaussian source profile along one axis."""
+    out = np.empty(cap, dtype=np.float32)
+    start = ctypes.c_int()
+    w = _L().oseh_gaussian_profile(float(sigma), int(center), int(n), ctypes.byref(start), out.ctypes.data_as(_fp),
+                                   cap)
+    if w < 1:
+        raise OseError(1, f"gaussian profile sigma={sigma} does not fit {cap} cells")
+    return start.value, out[:w].copy()
+
+
 SNAP_META = ("format", "lane", "step", "shot", "x0", "y0", "z0", "nx", "ny", "nz")
 
 

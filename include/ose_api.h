@@ -93,6 +93,16 @@ int ose_timed_run(ose_ctx* ctx, int nsteps, double* ms);
  * (ose_3d_kernel.cu:417-423).  Before the first ose_run; n = 0 restores
  * the plane.  Receivers outside the grid fail with OSE_EINVAL.            */
 int ose_set_receivers(ose_ctx* ctx, int shot_id, const int* xyz, int n);
+/* Spatially extended source for a shot, before the first ose_run: each
+ * step adds (s[it] + i h[it]) px[x-x0] py[y-y0] pz[z-z0] over the padded box
+ * box = {x0, y0, z0, wx, wy, wz} (widths 1..64, inside the grid) instead of
+ * the point of ose_add_shot (Xt_add_wavelet_sxyz ose_3d_lib.cu:610-633,
+ * add_wavelet_Xt ose_3d_kernel.cu:291-297).  A point injected every dt
+ * excites grid modes above the temporal Nyquist pi/dt, whose aliases spoil
+ * large time steps; a profile band-limited below it does not.  box = NULL
+ * restores the point.  No counterpart in the reference.                   */
+int ose_set_source_profile(ose_ctx* ctx, int shot_id, const int* box, const float* px,
+                           const float* py, const float* pz);
 /* Number of traces of a shot: its listed receivers, else nx*ny.           */
 int ose_receiver_count(ose_ctx* ctx, int shot_id, int* nrec);
 

@@ -125,6 +125,12 @@ int oseh_write_osnap(const char* path, int format, int lane, int step, int shot,
                      const void* lane_data);
 int oseh_read_osnap(const char* path, float* out, size_t cap, int* meta);
 
+/* Normalised Gaussian source profile along one axis: exp(-(j-c)^2/(2 s^2))
+ * over j in [c - ceil(4s), c + ceil(4s)] clipped to [0, n), summing to 1
+ * (s = 0: the unit point).  Writes the first index to *start and the
+ * values to out; returns the width, or -1 if it exceeds cap.              */
+int oseh_gaussian_profile(double sigma, int center, int n, int* start, float* out, int cap);
+
 /* ---- simulation ------------------------------------------------------ */
 /* A simulation is built from SU-style "key=value" parameters (the reference
  * hard-codes them in main(), ose_3d.cu:75-125) or a preset:
@@ -134,13 +140,19 @@ int oseh_read_osnap(const char* path, float* out, size_t cap, int* meta);
  * Keys: nx ny nz dx dy dz dt nt fm amp delay boundary bl br bf bb bu bd
  *       model seed v0 v1 iface vfile sx sy sz nshot jsx jsy jsz srcgrid
  *       rec_z korder ktol taper alpha extent hilbert device graph
- *       max_shots (cap on shots per model) snap segy
+ *       max_shots (cap on shots per model) snap segy src_sigma
  * Receivers: by default the whole interior plane rec_z.  ng=<n> records a
  * line (gxbeg + r*jgx, gybeg + r*jgy, gzbeg + r*jgz), r < n (the
  * reference's gxbeg..jgz, ng, ose_3d.cu:100-114; defaults jgx = 1,
  * jgy = jgz = 0); ngx/ngy record an ngx x ngy grid at depth gzbeg with
  * spacing jgx, jgy (default 1); csdgather=1 shifts every shot's receivers
- * by its source's offset from the model's first source.  Interior cells. */
+ * by its source's offset from the model's first source.  Interior cells.
+ * hilbert=unnorm|norm|analytic: Q's wavelet is nt H[s] (the reference),
+ * H[s], or -H[s] (the analytic signal of the exp(-i L dt) propagator).
+ * src_sigma=<cells>|auto replaces each point source by a normalised
+ * Gaussian profile (ose_set_source_profile); auto takes sigma = 3.72 v dt /
+ * (pi d) cells per axis, v the velocity at the source, so the profile's
+ * spectrum is below 1e-3 at the temporal Nyquist pi / (v dt).             */
 typedef struct oseh_sim oseh_sim;
 int  oseh_sim_create(oseh_sim** out, const char* params);
 /* Build velocities, pads, wavelets, orders and coefficients (host only).  */

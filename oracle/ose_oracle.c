@@ -431,6 +431,35 @@ done:
     return rc;
 }
 
+int orc_run_profile(const orc_model* m, int nt, const double* px, const double* py, const double* pz,
+                    const double* s, const double* h, int rz, int bl, int bf, int nx, int ny,
+                    double* traces, double* P, double* Q)
+{
+    const long sxy = (long)m->nxb * m->nyb;
+    for (int it = 0; it < nt; ++it) {
+        for (int z = 0; z < m->nzb; ++z) {
+            if (pz[z] == 0.0) continue;
+            for (int y = 0; y < m->nyb; ++y) {
+                if (py[y] == 0.0) continue;
+                const double a = py[y] * pz[z];
+                for (int x = 0; x < m->nxb; ++x) {
+                    if (px[x] == 0.0) continue;
+                    const long i = z * sxy + (long)y * m->nxb + x;
+                    P[i] += s[it] * (a * px[x]);
+                    Q[i] += h[it] * (a * px[x]);
+                }
+            }
+        }
+        if (orc_step(m, P, Q)) return -1;
+        if (traces)
+            for (int iy = 0; iy < ny; ++iy)
+                for (int ix = 0; ix < nx; ++ix)
+                    traces[((long)iy * nx + ix) * nt + it] =
+                        Q[rz * sxy + (long)(bf + iy) * m->nxb + bl + ix];
+    }
+    return 0;
+}
+
 int orc_run(const orc_model* m, int nt, long src, const double* s, const double* h,
             int rz, int bl, int bf, int nx, int ny, double* traces,
             double* P, double* Q)

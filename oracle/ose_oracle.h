@@ -25,6 +25,11 @@
  *                       -> IFFT -> scal_vr_3d    ose_3d_lib.cu:662-687
  *   orc_step            wavefield_extro_ori2       ose_3d_lib.cu:639-874
  *   orc_run             main() shot/time loop      ose_3d.cu:242-279
+ *   orc_run_profile     the same loop with a spatially extended source
+ *                       (not in the reference; a unit-vector profile is
+ *                       orc_run's point source, pinned bit for bit in
+ *                       tests/test_oracle.py, and superposition of two
+ *                       point runs pins the sum over a profile)
  *
  * Arrays are [z][y][x] with x fastest (ose_3d_kernel.cu:35).
  */
@@ -97,6 +102,17 @@ int orc_step(const orc_model* m, double* P, double* Q);
 int orc_run(const orc_model* m, int nt, long src, const double* s, const double* h,
             int rz, int bl, int bf, int nx, int ny, double* traces,
             double* P, double* Q);
+
+/* The same run with a spatially extended source: before each step
+ * P[z][y][x] += s[it] px[x] py[y] pz[z] and Q likewise with h (px, py, pz:
+ * full-length profiles, zero off the support).  Not in the reference, whose
+ * source is one cell (Xt_add_wavelet_sxyz, ose_3d_lib.cu:610-633): a point
+ * injected every dt excites the grid's modes above the temporal Nyquist
+ * pi/dt, a profile band-limited below it does not.  A unit-vector profile
+ * is orc_run's point source.                                            */
+int orc_run_profile(const orc_model* m, int nt, const double* px, const double* py, const double* pz,
+                    const double* s, const double* h, int rz, int bl, int bf, int nx, int ny,
+                    double* traces, double* P, double* Q);
 
 int orc_num_threads(void);
 

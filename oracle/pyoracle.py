@@ -48,6 +48,9 @@ def lib():
         L.orc_run.argtypes = [ctypes.POINTER(OrcModel), ctypes.c_int, ctypes.c_long, _dp, _dp,
                               ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                               _dp, _dp, _dp]
+        L.orc_run_profile.argtypes = [ctypes.POINTER(OrcModel), ctypes.c_int, _dp, _dp, _dp, _dp, _dp,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      _dp, _dp, _dp]
         L.orc_num_threads.restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -176,6 +179,21 @@ class Model:
                            _d(tr) if record else None, _d(P), _d(Q))
         assert rc == 0
         return tr, P, Q
+
+
+def run_profile(model, nt, px, py, pz, s, h, rz, bl, bf, nx, ny, P=None, Q=None, record=True):
+    """Model.run with the source P += s px py pz (full-length profiles)."""
+    m = model
+    shp = m.v.shape
+    P = np.zeros(shp) if P is None else _c(P).copy()
+    Q = np.zeros(shp) if Q is None else _c(Q).copy()
+    px, py, pz, s, h = _c(px), _c(py), _c(pz), _c(s), _c(h)
+    assert px.size == shp[2] and py.size == shp[1] and pz.size == shp[0]
+    tr = np.zeros((ny * nx, nt)) if record else None
+    rc = lib().orc_run_profile(ctypes.byref(m.s), nt, _d(px), _d(py), _d(pz), _d(s), _d(h), rz, bl, bf, nx, ny,
+                               _d(tr) if record else None, _d(P), _d(Q))
+    assert rc == 0
+    return tr, P, Q
 
 
 def num_threads():

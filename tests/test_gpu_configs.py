@@ -55,9 +55,9 @@ def test_trained_thin_pad_beats_plain_taper():
 def test_large_time_steps():
     """The step stays stable far beyond small R dt: from a smooth pulse next
     to the pad, with no source, 16 steps of 8 ms (R dt = 19.6, K = 36) only
-    lose energy.  With the reference's point injection per step the
-    seismogram at common times departs from the 0.5 ms run as dt grows (the
-    source, not the step operator, sets the error)."""
+    lose energy.  With a point injected per step the seismogram at common
+    times departs from the 0.5 ms run as dt grows (the source, not the step
+    operator, sets the error)."""
     import ose_amd
     from ose_amd import dtsweep
     from ose_amd import host as H
@@ -81,8 +81,8 @@ def test_large_time_steps():
     print("energies", e, flush=True)
     assert all(b <= a * (1 + 1e-4) for a, b in zip(e, e[1:]))
     print("dt sweep 64^3", flush=True)
-    rows = dtsweep.sweep(n=64, T=0.256, dts=(0.5e-3, 1e-3, 2e-3), log=lambda m: print(m, flush=True))
+    rows = dtsweep.sweep(n=64, T=0.224, dts=(0.5e-3, 1e-3, 2e-3, 8e-3), log=lambda m: print(m, flush=True))
     ks = [r["K"] for r in rows]
     err = [r["rel_l2_vs_smallest_dt"] for r in rows]
-    assert ks[0] < ks[1] < ks[2]
-    assert err[0] == 0.0 and err[1] < 0.1 and err[1] < err[2] < 0.3
+    assert ks[0] < ks[1] < ks[2] < ks[3]
+    assert err[0] == 0.0 and err[1] < 0.02 and err[1] < err[2] < 0.05 and err[3] > 0.5

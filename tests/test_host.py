@@ -323,3 +323,19 @@ def test_snapshot_codec_and_osnap_files(H, tmp_path, fmt):
         bad = sc.encode(fmt, [q])[:4 * ny * nz] + sc.encode(fmt, [p])[4 * ny * nz:]
         good = sc.encode(fmt, [p])
         assert not np.array_equal(H.decode_snapshot(fmt, nx, ny, nz, bad), H.decode_snapshot(fmt, nx, ny, nz, good))
+
+
+def test_gaussian_source_profile(H):
+    """Normalised, symmetric, +-ceil(4 sigma) wide, clipped at the grid edge;
+    sigma 0 is the unit point."""
+    a, p = H.gaussian_profile(2.5, 30, 64)
+    assert a == 20 and p.size == 21
+    assert abs(float(p.sum(dtype=np.float64)) - 1.0) < 1e-6
+    assert np.allclose(p, p[::-1]) and p.argmax() == 10
+    assert np.isclose(p[10 + 3] / p[10], np.exp(-0.5 * (3 / 2.5) ** 2), rtol=1e-5)
+    a, p = H.gaussian_profile(2.5, 3, 64)
+    assert a == 0 and p.size == 14 and abs(float(p.sum(dtype=np.float64)) - 1.0) < 1e-6
+    a, p = H.gaussian_profile(0.0, 7, 16)
+    assert a == 7 and np.array_equal(p, [1.0])
+    with pytest.raises(H.OseError):
+        H.gaussian_profile(9.0, 40, 128)      # 73 cells > 64

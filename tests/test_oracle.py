@@ -187,3 +187,26 @@ def test_resume_from_state_is_bit_identical():
     tr2, P2, Q2 = m.run(n - h, *args, t["s"][h:], t["h"][h:], int(t["rz"]), bl, bf, nx, ny, P=P1, Q=Q1)
     assert np.array_equal(np.concatenate([tr1, tr2], axis=1), tr)
     assert np.array_equal(P2, P) and np.array_equal(Q2, Q)
+
+
+def test_profile_source_pins_to_point_runs():
+    """A unit-vector profile is the point source (bit for bit); a profile
+    with two nonzero x entries equals the weighted sum of two point runs
+    (linearity of the step)."""
+    t = np.load(os.path.join(GOLD, "step_tiny.npz"))
+    nx, ny, nz, bl, br, bf, bb, bu, bd, nxb, nyb, nzb = (int(x) for x in t["dims"])
+    m = orc.Model(t["v"], *t["d"], float(t["R"]), t["coef"], t["gx"], t["gy"], t["gz"])
+    n = min(t["s"].size, 6)
+    s, h = t["s"][:n], t["h"][:n]
+    src = int(t["src"])
+    sz, sy, sx = src // (nxb * nyb), (src // nxb) % nyb, src % nxb
+    e = [np.zeros(k) for k in (nxb, nyb, nzb)]
+    e[0][sx], e[1][sy], e[2][sz] = 1.0, 1.0, 1.0
+    tr, P, Q = m.run(n, src, s, h, int(t["rz"]), bl, bf, nx, ny)
+    tp, Pp, Qp = orc.run_profile(m, n, e[0], e[1], e[2], s, h, int(t["rz"]), bl, bf, nx, ny)
+    assert np.array_equal(tp, tr) and np.array_equal(Pp, P) and np.array_equal(Qp, Q)
+    px = np.zeros(nxb)
+    px[sx], px[sx - 2] = 0.75, -0.5
+    tp, Pp, Qp = orc.run_profile(m, n, px, e[1], e[2], s, h, int(t["rz"]), bl, bf, nx, ny)
+    _, P2, Q2 = m.run(n, src - 2, s, h, int(t["rz"]), bl, bf, nx, ny)
+    assert rel(Pp + 1j * Qp, 0.75 * (P + 1j * Q) - 0.5 * (P2 + 1j * Q2)) < 1e-12
