@@ -183,7 +183,8 @@ int  oseh_sim_write(oseh_sim* sim, const char* dir);
  * snap_region=padded|interior, snap_lanes=pq|p|q.  Per lane and shot,
  * float32 goes to dir/p-<it>-shot<k>.bin in the snapshot order below, the
  * encoded formats to dir/p-<it>-shot<k>.osnap (q- for Im u).  A failed
- * write is reported by the next oseh_sim_run or oseh_sim_write.           */
+ * write is reported by the next oseh_sim_run that finds it, at the latest
+ * by oseh_sim_write (which waits for every snapshot).                     */
 int  oseh_sim_stream_snapshots(oseh_sim* sim, const char* dir, int every);
 /* Write the current wavefield of every shot as p-<it>-shot<k>.bin (Re u)
  * and q-<it>-shot<k>.bin (Im u) in the snapshot order (ose_3d.cu:260-270). */

@@ -139,3 +139,19 @@ def test_driver_streams_q8_interior(tmp_path):
     assert (np.abs(got - ref) <= sc.bound(sc.Q8, ref)).all()
     assert np.abs(ref).max() > 0
     assert (tmp_path / "q-10-shot0.osnap").exists() and not (tmp_path / "p-10-shot0.osnap").exists()
+
+
+def test_failed_snapshot_write_is_reported(tmp_path):
+    """A snapshot the writer thread cannot store is reported (by the next
+    run call, or at the latest by write, which drains the stream) with the
+    file named, instead of being dropped."""
+    from ose_amd import host as H
+    print("snapshot write failure", flush=True)
+    sim = H.Simulation("preset=ref nt=10").setup()
+    try:
+        sim.stream_snapshots(tmp_path / "missing" / "dir", 5)
+        with pytest.raises(H.OseError, match="cannot write snapshot"):
+            sim.run(10)
+            sim.write(str(tmp_path))
+    finally:
+        sim.close()

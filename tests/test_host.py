@@ -339,3 +339,21 @@ def test_gaussian_source_profile(H):
     assert a == 7 and np.array_equal(p, [1.0])
     with pytest.raises(H.OseError):
         H.gaussian_profile(9.0, 40, 128)      # 73 cells > 64
+
+
+def test_osnap_reader_rejects_bad_files(H, tmp_path):
+    import _snapcodec as sc
+    v = np.arange(2 * 3 * 16, dtype=np.float32).reshape(2, 3, 16)
+    raw = sc.encode(sc.Q8, [v])
+    good = tmp_path / "g.osnap"
+    box = (ctypes.c_int * 6)(0, 0, 0, 16, 3, 2)
+    assert H._L().oseh_write_osnap(str(good).encode(), sc.Q8, 0, 1, 0, box, raw) == 0
+    data = open(good, "rb").read()
+    for name, blob in (("magic", b"XSESNAP1" + data[8:]), ("short", data[:-1]), ("long", data + b"\0"),
+                       ("size", data[:48] + (123).to_bytes(8, "little") + data[56:])):
+        p = tmp_path / f"{name}.osnap"
+        open(p, "wb").write(blob)
+        with pytest.raises(H.OseError):
+            H.read_osnap(p)
+    got, meta = H.read_osnap(good)
+    assert got.shape == (2, 3, 16) and meta["format"] == sc.Q8
