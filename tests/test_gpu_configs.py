@@ -37,6 +37,31 @@ def test_c0_seismogram_matches_fp64_oracle():
     assert abs(ratio - 1.0) < 1e-4
 
 
+def test_c1_seismogram_matches_fp64_oracle_at_full_size():
+    """256^3 layered model with anomalies, 40-cell taper, 336^3 padded, K =
+    12: the first 100 steps of the FP32 GPU run against the FP64 oracle
+    (every 8th receiver in x and y, tests/golden/gen_c1_oracle.py)."""
+    from ose_amd.host import Simulation
+    path = os.path.join(GOLD, "c1_oracle.npz")
+    if not os.path.exists(path):
+        pytest.skip("c1_oracle.npz not generated (tests/golden/gen_c1_oracle.py, about an hour of FP64 CPU)")
+    g = np.load(path)
+    nt = int(g["nt"])
+    print(f"c1 {nt} steps", flush=True)
+    sim = Simulation(f"preset=c1 nt={nt}").setup()
+    try:
+        assert sim.model(0)["K"] == int(g["K"])
+        sim.run(nt)
+        tr = sim.traces(0, 0, nt)
+    finally:
+        sim.close()
+    err = rel(tr[g["idx"]], g["traces"])
+    ratio = np.linalg.norm(tr.astype(np.float64)) / float(g["norm_full"])
+    print(f"c1 rel-L2 (every 8th receiver) {err:.3g}; full-array norm ratio {ratio:.7f}", flush=True)
+    assert err < 1e-4
+    assert abs(ratio - 1.0) < 1e-4
+
+
 def test_trained_thin_pad_beats_plain_taper():
     """Held-out seeded model (64^3 interior): the committed 8-cell learned
     pad against the 8-cell reference taper, both measured against a 64-cell

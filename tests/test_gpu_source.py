@@ -82,3 +82,26 @@ def test_band_limited_source_allows_large_time_steps():
     assert rows[2]["K"] >= 30 and rows[2]["src_sigma_cells"] > 1.0
     point = D.sweep(n=128, T=0.224, dts=(1e-3, 8e-3), log=lambda m: None)
     assert point[1]["rel_l2_vs_smallest_dt"] > 0.5
+
+
+def test_driver_large_step_keys(tmp_path):
+    """ose3d with 4 ms steps, an automatic source profile and the analytic
+    wavelet pair: the order follows R dt and the run completes."""
+    import json
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "3d-ose_amd", "bin", "ose3d")
+    print("ose3d dt=0.004 src_sigma=auto hilbert=analytic", flush=True)
+    # the Ricker is delayed 40 samples plus 1/fm: it peaks near step 48 at 4 ms
+    p = subprocess.run([exe, "preset=ref", "nt=90", "dt=0.004", "src_sigma=auto", "hilbert=analytic",
+                        f"out={tmp_path}"], capture_output=True, text=True, timeout=240)
+    assert p.returncode == 0, p.stderr
+    out = json.loads(p.stdout.strip().splitlines()[-1])
+    mod = out["setup"]["models"][0]
+    assert out["steps"] == 90 and mod["K"] > mod["R"] * 0.004 and mod["K"] >= 12
+    tr = np.fromfile(tmp_path / "shot0.bin", dtype=np.float32)
+    assert tr.size == 101 * 101 * 90 and np.all(np.isfinite(tr)) and np.abs(tr).max() > 0
+    bad = subprocess.run([exe, "preset=ref", "nt=5", "hilbert=upside", f"out={tmp_path}"], capture_output=True,
+                         text=True, timeout=120)
+    assert bad.returncode != 0
