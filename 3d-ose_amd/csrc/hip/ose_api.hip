@@ -470,13 +470,18 @@ int launch_strided(ose_ctx* c, int kind, const StridedKern& k, const float2* in,
     a.os = zaxis ? (long)nx : (long)nx * ny;
     a.shot_stride = shot_stride;
     a.tw = tw;
+    dim3 grid((nx + k.T - 1) / k.T, zaxis ? ny : nz, nshot);
+    // XCD-contiguous tile order (StridedTile); its exact division by the
+    // grid extents holds for < 2^24 workgroups and extents < 2^16
+    a.xcd = (double)grid.x * grid.y * grid.z < 16777216.0 && grid.x < 65536 && grid.y < 65536;
+    a.mgx = (1ULL << 40) / grid.x + 1;
+    a.mgy = (1ULL << 40) / grid.y + 1;
     if (m) {
         a.kx2 = m->d_kx2;
         a.ky2 = m->d_ky2;
         a.kz2 = m->d_kz2;
         a.invN = (float)(1.0 / ((double)nx * ny * nz));
     }
-    dim3 grid((nx + k.T - 1) / k.T, zaxis ? ny : nz, nshot);
     return launch(c, kind, bytes, k.fn, grid, dim3(k.threads), k.lds, &a);
 }
 

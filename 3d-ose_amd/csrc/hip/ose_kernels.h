@@ -59,6 +59,46 @@ struct StridedArgs {
     const float* ky2;
     const float* kz2;
     float invN;         // 1 / (nx ny nz)
+    int xcd;            // 1: XCD-contiguous tile order (StridedTile)
+    unsigned long long mgx, mgy;   // 2^40 / gridDim.x + 1 and 2^40 / gridDim.y + 1
+};
+
+// floor(n / d) as (n * m) >> 40 with m = 2^40 / d + 1: exact for n < 2^24 and
+// d < 2^16 (the error n (m - 2^40/d) / 2^40 < n / 2^40 stays below 1/d), and
+// all in scalar registers for wave-uniform n (a division by a runtime value
+// would be expanded into vector reciprocal code).
+__device__ __forceinline__ unsigned div_magic(unsigned n, unsigned long long m)
+{
+    return (unsigned)(((unsigned long long)n * m) >> 40);
+}
+
+// The workgroup's tile (column block bx, row by, shot bz).  Workgroups are
+// dispatched round-robin over the chip's 8 XCDs by linear id; with xcd set,
+// XCD x takes the contiguous tile range [x q + min(x, r), ...) of the
+// q = n / 8, r = n % 8 split, column block fastest, so the workgroups that
+// run together on one XCD hold neighbouring column blocks of the same rows
+// (one set of pages, whole 256-byte segments) instead of blocks 8 apart.
+struct StridedTile {
+    int bx, by, bz;
+    __device__ __forceinline__ explicit StridedTile(const StridedArgs& a)
+    {
+        if (!a.xcd) {
+            bx = blockIdx.x;
+            by = blockIdx.y;
+            bz = blockIdx.z;
+            return;
+        }
+        const unsigned gx = gridDim.x, gy = gridDim.y;
+        const unsigned n = gx * gy * gridDim.z;
+        const unsigned id = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+        const unsigned x = id % 8, q = n / 8, r = n % 8;
+        const unsigned tile = x * q + min(x, r) + id / 8;
+        const unsigned row = div_magic(tile, a.mgx);
+        const unsigned shot = div_magic(row, a.mgy);
+        bx = (int)(tile - row * gx);
+        by = (int)(row - shot * gy);
+        bz = (int)shot;
+    }
 };
 
 // SPLIT = 1 exchanges the real and then the imaginary parts through an LDS
@@ -77,40 +117,57 @@ struct StridedCfg {
 // i < NW at row wrow(i), readers (rd) load v[i] for i < NR from row
 // rrow(i).  In split form a thread that both writes and reads overwrites
 // only the real parts of v[0..NR) before it stores its imaginary parts.
+//
+// LDS row of access i.  One side of every exchange strides across all of it,
+// and ds instructions carry at most a 64 KiB immediate offset: beyond that
+// (T = 32 at 864: 110 KiB) the compiler gives each far access its own
+// address register, which spills.  So where the exchange exceeds 64 KiB the
+// upper half of a loop addresses from row(n/2), passed through an empty asm
+// statement so that it stays one base register with immediate offsets.
+template <bool FAR, int NI, typename FROW>
+__device__ __forceinline__ int xrow(FROW row, int i)
+{
+    if (!FAR || i < NI / 2) return row(i);
+    int mid = row(NI / 2);
+    asm("" : "+v"(mid));
+    return mid + (row(i) - row(NI / 2));
+}
+
 template <int T, int SPLIT, int NW, int NR, typename FW, typename FR>
 __device__ __forceinline__ void col_exchange(float2* v, bool wr, bool rd, FW wrow, FR rrow, float2* lds, int col)
 {
     constexpr int LDT = T + 1;
+    constexpr bool FAR = (size_t)NW * NR * LDT * (SPLIT ? sizeof(float) : sizeof(float2)) > 65536;
     if (!SPLIT) {
         if (wr) {
 #pragma unroll
-            for (int i = 0; i < NW; ++i) lds[wrow(i) * LDT + col] = v[i];
+            for (int i = 0; i < NW; ++i) lds[xrow<FAR, NW>(wrow, i) * LDT + col] = v[i];
         }
         __syncthreads();
         if (rd) {
 #pragma unroll
-            for (int i = 0; i < NR; ++i) v[i] = lds[rrow(i) * LDT + col];
+            for (int i = 0; i < NR; ++i) v[i] = lds[xrow<FAR, NR>(rrow, i) * LDT + col];
         }
     } else {
         float* f = reinterpret_cast<float*>(lds);
         if (wr) {
 #pragma unroll
-            for (int i = 0; i < NW; ++i) f[wrow(i) * LDT + col] = v[i].x;
+            for (int i = 0; i < NW; ++i) f[xrow<FAR, NW>(wrow, i) * LDT + col] = v[i].x;
         }
         __syncthreads();
         if (rd) {
 #pragma unroll
-            for (int i = 0; i < NR; ++i) v[i].x = f[rrow(i) * LDT + col];
+            for (int i = 0; i < NR; ++i) v[i].x = f[xrow<FAR, NR>(rrow, i) * LDT + col];
         }
         __syncthreads();
         if (wr) {
 #pragma unroll
-            for (int i = 0; i < NW; ++i) f[wrow(i) * LDT + col] = v[i].y;
+            for (int i = 0; i < NW; ++i) f[xrow<FAR, NW>(wrow, i) * LDT + col] = v[i].y;
         }
         __syncthreads();
         if (rd) {
 #pragma unroll
-            for (int i = 0; i < NR; ++i) v[i].y = f[rrow(i) * LDT + col];
+            for (int i = 0; i < NR; ++i) v[i].y = f[xrow<FAR, NR>(rrow, i) * LDT + col];
         }
     }
 }
@@ -128,13 +185,13 @@ struct StridedAddr {
     int xc;            // column actually addressed
     bool ok;           // the thread's own column exists
 
-    __device__ __forceinline__ StridedAddr(const StridedArgs& a, int T, int col, int t)
+    __device__ __forceinline__ StridedAddr(const StridedArgs& a, const StridedTile& g, int T, int col, int t)
     {
-        const int x0 = blockIdx.x * T;
+        const int x0 = g.bx * T;
         xc = min(col, a.ncol - 1 - x0);
         ok = col == xc;
         es = a.es;
-        wbase = (long)blockIdx.z * a.shot_stride + (long)blockIdx.y * a.os + x0;
+        wbase = (long)g.bz * a.shot_stride + (long)g.by * a.os + x0;
         boff = ((unsigned)xc + (unsigned)t * (unsigned)a.es) * (unsigned)sizeof(float2);
     }
     typedef __attribute__((address_space(1))) char gchar;
@@ -173,7 +230,8 @@ void k_strided(StridedArgs a)
     const int tid = threadIdx.x;
     const int col = tid % T;
     const int t = tid / T;
-    const StridedAddr ad(a, T, col, t);
+    const StridedTile g(a);
+    const StridedAddr ad(a, g, T, col, t);
     float2 v[C::NT];
     if (t < N2) {
 #pragma unroll
@@ -199,7 +257,7 @@ void k_strided(StridedArgs a)
 
 // z pass: forward-z, multiply by |k|/N (folded wavenumbers, scale_kxkz_3D
 // ose_3d_kernel.cu:101-126 and the 1/N of scal_vr_3d :355), inverse-z.
-// In place; blockIdx.y is the (spectral) y index.  Each LDS location of the
+// In place; the tile row (StridedTile::by) is the (spectral) y index.  Each LDS location of the
 // second exchange is written by the thread that read it in the first, so
 // the exchanges need no barrier between them.
 // The split form asks for at least 4 waves per SIMD (at most 128 VGPRs):
@@ -215,7 +273,8 @@ void k_zsym(StridedArgs a)
     const int tid = threadIdx.x;
     const int col = tid % T;
     const int t = tid / T;
-    const StridedAddr ad(a, T, col, t);
+    const StridedTile g(a);
+    const StridedAddr ad(a, g, T, col, t);
     float2 v[C::NT];
     if (t < N2) {
 #pragma unroll
@@ -229,7 +288,7 @@ void k_zsym(StridedArgs a)
         v, t < N2, t < N1, [&](int k1) { return t * N1 + k1; }, [&](int n2) { return n2 * N1 + t; }, lds, col);
     if (t < N1) {
         rdft<N2, -1>(v);
-        const float kxy = a.kx2[blockIdx.x * T + ad.xc] + a.ky2[blockIdx.y];
+        const float kxy = a.kx2[g.bx * T + ad.xc] + a.ky2[g.by];
 #pragma unroll
         for (int k2 = 0; k2 < N2; ++k2) {
             // hardware square root (1 ulp): |k| needs no correctly rounded
@@ -413,7 +472,12 @@ void k_xpass(XArgs a)
     extern __shared__ float2 lds[];
     const int l = threadIdx.x / C::NT;
     const int t = threadIdx.x % C::NT;
-    const long line = (long)blockIdx.x * P + l;
+    // Line block: the hardware deals workgroups round-robin over 8 XCDs, so
+    // block b of a shot runs on the XCD of b % 8; that XCD takes the
+    // contiguous range starting at (b % 8) q + min(b % 8, r) (q = blocks / 8,
+    // r = blocks % 8) instead of every 8th block.
+    const unsigned bx = blockIdx.x, bq = gridDim.x / 8, br = gridDim.x % 8;
+    const long line = (long)((bx % 8) * bq + min(bx % 8, br) + bx / 8) * P + l;
     const bool okl = line < a.nlines;
     const int y = okl ? (int)(line % a.ny) : 0;
     const int z = okl ? (int)(line / a.ny) : 0;
