@@ -44,7 +44,7 @@ def strided_variants(n1, n2):
     nt = max(n1, n2)
     out = []
     for t in STRIDED_TS:
-        if t * nt > 768:      # larger workgroups spill the register DFTs
+        if t * nt > 896:      # larger workgroups spill the register DFTs (T = 32 needs N1, N2 <= 28)
             continue
         full = 8 * n1 * n2 * (t + 1)
         if full <= LDS_MAX:
