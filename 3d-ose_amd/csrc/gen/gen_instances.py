@@ -106,6 +106,9 @@ XPASS = """    {{
         k.fn[XM_FWD] = (const void*)&k_xpass<N1, N2, P, XM_FWD, SPLIT, TWB>;
         k.fn[XM_INV] = (const void*)&k_xpass<N1, N2, P, XM_INV, SPLIT, TWB>;
         k.fn[XM_LASTA] = (const void*)&k_xpass<N1, N2, P, XM_LASTA, SPLIT, TWB>;
+        k.fn[XM_SKIP] = (const void*)&k_xpass<N1, N2, P, XM_SKIP, SPLIT, TWB>;
+        k.fn[XM_ORDER0N] = (const void*)&k_xpass<N1, N2, P, XM_ORDER0N, SPLIT, TWB>;
+        k.fn[XM_MIDF] = (const void*)&k_xpass<N1, N2, P, XM_MIDF, SPLIT, TWB>;
         e.xk[e.nx++] = k;
     }}
 """

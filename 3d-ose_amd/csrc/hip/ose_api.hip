@@ -195,7 +195,103 @@ using namespace ose;
 // ---------------------------------------------------------------------
 namespace {
 
-enum PassKind { PK_YF = 0, PK_Z = 1, PK_YI = 2, PK_XMID = 3, PK_X0 = 4, PK_XLAST = 5, PK_INJ = 6, PK_N = 7 };
+enum PassKind {
+    PK_YF = 0, PK_Z = 1, PK_YI = 2, PK_XMID = 3, PK_X0 = 4, PK_XLAST = 5, PK_INJ = 6, PK_XSKIP = 7, PK_N = 8
+};
+
+// Which orders of a step touch the accumulator (see the deferred
+// accumulation of k_xpass).  Pass 0 holds W_0 (an extra read) and W_1;
+// pass j >= 1 holds W_{j-1} and W_{j+1}; the last pass, K-1, always touches
+// (it forms u).  Every term W_m, m = 0..K, is added exactly once, by one of
+// the passes that hold it.  The set of touching passes minimises the step's
+// x-pass bytes per cell (dynamic programming over the passes; a touch costs
+// the accumulator's read and write, 16 B, or 8 B for the first, which only
+// writes), e.g. K = 11 touches passes 1 2 5 6 9 10: 468 B/cell of x passes
+// instead of 556.
+struct AccPlan {
+    bool touch = false;
+    bool add_prev = false, add_next = false;   // W_{j-1} (W_0 at pass 0), W_{j+1}
+    int ea = -1, eb = 0, ec = 0;               // powers of g (XArgs)
+};
+
+double x_pass_bytes(int j, int K, bool touch, bool stored)
+{
+    if (j == K - 1) return stored ? 44.0 : 36.0;      // S, W_{K-2}, vr (+A) read; u, X_0 written
+    if (j == 0) return touch ? 44.0 : 28.0;           // S, vr (+W_0) read; W_1, X_1 (+A) written
+    return touch ? (stored ? 52.0 : 44.0) : 36.0;     // S, W_{j-1}, vr (+A) read; W_{j+1}, X_{j+1} (+A) written
+}
+
+std::vector<AccPlan> plan_accumulation(int K)
+{
+    // state after deciding pass j: (touch j-1, touch j, any touch so far)
+    struct Cell { double cost = -1; std::vector<char> s; };
+    Cell dp[2][2][2];
+    for (int s0 = 0; s0 < 2; ++s0) {
+        if (K == 1 && !s0) continue;
+        dp[0][s0][s0].cost = x_pass_bytes(0, K, s0, false);
+        dp[0][s0][s0].s = {(char)s0};
+    }
+    for (int j = 1; j < K; ++j) {
+        Cell nd[2][2][2];
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                for (int any = 0; any < 2; ++any) {
+                    const Cell& c = dp[a][b][any];
+                    if (c.cost < 0) continue;
+                    for (int sj = (j == K - 1); sj < 2; ++sj) {
+                        // W_{j-1} is held by passes j-2 and j (W_0: passes 0 and 1)
+                        const bool covered = (j == 1) ? (b || sj) : (a || sj);
+                        if (!covered) continue;
+                        const double cost = c.cost + x_pass_bytes(j, K, sj, any);
+                        Cell& n = nd[b][sj][any || sj];
+                        if (n.cost < 0 || cost < n.cost) {
+                            n.cost = cost;
+                            n.s = c.s;
+                            n.s.push_back((char)sj);
+                        }
+                    }
+                }
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                for (int any = 0; any < 2; ++any) dp[a][b][any] = nd[a][b][any];
+    }
+    const Cell* best = nullptr;
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int any = 0; any < 2; ++any) {
+                const Cell& c = dp[a][b][any];
+                // W_{K-1} is held only by pass K-2
+                if (c.cost < 0 || (K >= 2 && !c.s[K - 2])) continue;
+                if (!best || c.cost < best->cost) best = &c;
+            }
+    std::vector<AccPlan> plan(K);
+    if (!best) return plan;   // unreachable for K >= 1
+    for (int j = 0; j < K; ++j) plan[j].touch = best->s[j];
+    // assign W_m to the pass that writes it if that pass touches, else to
+    // the pass that reads it (W_0: pass 0, else pass 1)
+    for (int m = 0; m <= K; ++m) {
+        const int p = (m == 0) ? (plan[0].touch ? 0 : 1) : (plan[m - 1].touch ? m - 1 : m + 1);
+        if (m == 0 && p == 0) plan[0].add_prev = true;
+        else if (p == m - 1) plan[p].add_next = true;
+        else plan[p].add_prev = true;
+    }
+    // taper applications of W_m's term by the end of order j in the
+    // reference: W_0, W_1 are added at order 0 and tapered at orders 1..j;
+    // W_m (m >= 2) is added at order m-1 and tapered at orders m-1..j
+    auto napp = [](int m, int j) { return m <= 1 ? j : j - m + 2; };
+    int prev = -1;
+    for (int j = 0; j < K; ++j) {
+        if (!plan[j].touch) continue;
+        plan[j].ea = prev < 0 ? -1 : j - prev;
+        plan[j].eb = j == 0 ? 0 : napp(j - 1, j);
+        plan[j].ec = napp(j + 1, j);
+        prev = j;
+    }
+    // the kernels raise g to powers 0..3 only (gpow)
+    for (int j = 0; j < K; ++j)
+        if (plan[j].touch && (plan[j].ea > 3 || plan[j].eb > 3 || plan[j].ec > 3 || plan[j].eb < 0)) return {};
+    return plan;
+}
 
 struct DevBuf {
     void* p = nullptr;
@@ -205,6 +301,7 @@ struct DevBuf {
 struct Model {
     ose_model_desc d{};
     std::vector<double> coef;
+    std::vector<AccPlan> acc_plan;   // per order of a step
     std::vector<float> gx, gy, gz;
     const FftEntry *ex = nullptr, *ey = nullptr, *ez = nullptr;
     // chosen launch shapes (index into the entry's variants)
@@ -496,7 +593,10 @@ int launch_x(ose_ctx* c, int kind, const XKern& k, int mode, XArgs& a, int nline
     case XM_FWD: ok = ok && a.A && a.Xout; break;
     case XM_INV: ok = ok && a.A && a.Xprev; break;
     case XM_ORDER0: ok = ok && a.S && a.Win && a.Wout && a.Xout && a.A && a.vr; break;
-    case XM_MID: ok = ok && a.S && a.Win && a.Wout && a.Xout && a.A && a.vr && a.gx && a.gy && a.gz; break;
+    case XM_ORDER0N: ok = ok && a.S && a.Wout && a.Xout && a.vr; break;
+    case XM_SKIP: ok = ok && a.S && a.Win && a.Wout && a.Xout && a.vr; break;
+    case XM_MID:
+    case XM_MIDF: ok = ok && a.S && a.Win && a.Wout && a.Xout && a.A && a.vr && a.gx && a.gy && a.gz; break;
     case XM_LAST:
         ok = ok && a.S && a.Win && a.Wout && a.Xout && a.A && a.vr && a.gx && a.gy && a.gz && a.traces && a.it;
         break;
@@ -540,6 +640,11 @@ XArgs base_xargs(const Model& m, int nshot)
     a.shot_stride = m.ncell;
     a.ny = m.d.nyb;
     a.inv_nx = (float)(1.0 / (double)m.d.nxb);
+    // MID/LAST default to the reference's per-order update A <- g (A + c1 W_{j+1})
+    a.ea = 1;
+    a.eb = 0;
+    a.ec = 1;
+    a.cp = 0.f;
     (void)nshot;
     return a;
 }
@@ -605,20 +710,24 @@ int enqueue_model_step(ose_ctx* c, Model& m, int* d_it)
         a.S = m.X;
         a.A = m.A;
         a.Xout = m.X;
+        const AccPlan& pl = m.acc_plan[j];
         int mode, kind;
-        double bytes;
+        double bytes = x_pass_bytes(j, K, pl.touch, pl.ea >= 0) * cells;
         if (j == 0) {
-            mode = XM_ORDER0;
+            mode = pl.touch ? XM_ORDER0 : XM_ORDER0N;
             kind = PK_X0;
             a.Win = m.W[0];
             a.Wout = m.W[1];
-            a.c0 = (float)m.coef[0];
-            a.c1 = (float)m.coef[1];
-            bytes = 44.0 * cells;   // S, W0, vr read; W1, X1, A written
+            a.c0 = pl.add_prev ? (float)m.coef[0] : 0.f;
+            a.c1 = pl.add_next ? (float)m.coef[1] : 0.f;
         } else {
             // W_{j-1} lives in W[(j-1)%2] and W_{j+1} replaces it
             a.Win = a.Wout = m.W[(j - 1) % 2];
-            a.c1 = (float)m.coef[j + 1];
+            a.c1 = pl.add_next ? (float)m.coef[j + 1] : 0.f;
+            a.cp = pl.add_prev ? (float)m.coef[j - 1] : 0.f;
+            a.ea = pl.ea;
+            a.eb = pl.eb;
+            a.ec = pl.ec;
             if (j == K - 1) {
                 mode = m.learned_on ? XM_LASTA : XM_LAST;
                 kind = PK_XLAST;
@@ -632,12 +741,13 @@ int enqueue_model_step(ose_ctx* c, Model& m, int* d_it)
                 a.rec_nx = m.rec_nx;
                 a.it = d_it;
                 a.nt_cap = m.nt_cap;
-                // S, W_{K-2}, vr, A read; u and (not LASTA) X0 written
-                bytes = (m.learned_on ? 36.0 : 44.0) * cells;
-            } else {
-                mode = XM_MID;
+                if (m.learned_on) bytes -= 8.0 * cells;   // LASTA writes no X_0
+            } else if (pl.touch) {
+                mode = pl.ea >= 0 ? XM_MID : XM_MIDF;
                 kind = PK_XMID;
-                bytes = 52.0 * cells;   // S, W_{j-1}, vr, A read; W_{j+1}, X_{j+1}, A written
+            } else {
+                mode = XM_SKIP;
+                kind = PK_XSKIP;
             }
         }
         if ((rc = launch_x(c, kind, m.ex->xk[m.sel_x], mode, a, ny * nz, nshot, bytes))) return rc;
@@ -733,6 +843,12 @@ int autotune_model(ose_ctx* c, Model& m)
                                                          m.ncell, m.d_twy, nullptr, 0.0); }, reps, &t);
         if (!rc && t < best) best = t, m.sel_yi = v;
     }
+    // x pass: the step's middle orders either update the accumulator (MID)
+    // or leave it alone (SKIP); weigh each variant's two times by how often
+    // the step runs each mode
+    int n_mid = 0, n_skip = 0;
+    for (int j = 1; j + 1 < m.d.K; ++j) (m.acc_plan[j].touch ? n_mid : n_skip)++;
+    if (n_mid + n_skip == 0) n_mid = 1;
     best = 1e30f;
     for (int v = 0; v < m.ex->nx && !rc; ++v) {
         XArgs a = base_xargs(m, nshot);
@@ -741,8 +857,13 @@ int autotune_model(ose_ctx* c, Model& m)
         a.Win = a.Wout = m.W[1];
         a.Xout = m.X;
         a.c1 = 0.5f;
+        float ts = 0.f;
         rc = time_variant(c, [&] { return launch_x(c, -1, m.ex->xk[v], XM_MID, a, ny * nz, nshot, 0.0); }, reps, &t);
-        if (!rc && t < best) best = t, m.sel_x = v;
+        if (!rc && n_skip)
+            rc = time_variant(c, [&] { return launch_x(c, -1, m.ex->xk[v], XM_SKIP, a, ny * nz, nshot, 0.0); },
+                              reps, &ts);
+        const float tw = n_mid * t + n_skip * ts;
+        if (!rc && tw < best) best = tw, m.sel_x = v;
     }
     c->profile = prof;
     if (!rc) m.tuned = true;
@@ -1231,6 +1352,23 @@ const char* ose_last_error(const ose_ctx* c)
 }
 
 int ose_supported_extent(int n) { return next_supported(n); }
+
+int ose_accumulation_plan(int K, int* plan)
+{
+    if (!plan || K < 2 || K > 64) return OSE_EINVAL;
+    const std::vector<AccPlan> p = plan_accumulation(K);
+    if ((int)p.size() != K) return OSE_EINVAL;
+    for (int j = 0; j < K; ++j) {
+        int* o = plan + 6 * j;
+        o[0] = p[j].touch;
+        o[1] = p[j].add_prev;
+        o[2] = p[j].add_next;
+        o[3] = p[j].ea;
+        o[4] = p[j].eb;
+        o[5] = p[j].ec;
+    }
+    return OSE_OK;
+}
 int ose_extent_is_supported(int n) { return find_entry(n) != nullptr; }
 
 int ose_add_model(ose_ctx* c, const ose_model_desc* d, const float* v, int* model_id)
@@ -1257,6 +1395,9 @@ int ose_add_model(ose_ctx* c, const ose_model_desc* d, const float* v, int* mode
                        next_supported(d->nzb));
     m->d = *d;
     m->coef.assign(d->coef, d->coef + d->K + 1);
+    m->acc_plan = plan_accumulation(d->K);
+    if ((int)m->acc_plan.size() != d->K)
+        return c->fail(OSE_EINVAL, "ose_add_model: no accumulation plan for K = %d", d->K);
     for (double x : m->coef)
         if (!std::isfinite(x)) return c->fail(OSE_ENUMERIC, "ose_add_model: non-finite coefficient");
     m->gx.assign(d->taper_x, d->taper_x + d->nxb);

@@ -18,7 +18,7 @@ struct StridedKern {
 struct XKern {
     int P = 0, split = 0, twb = 0, threads = 0;
     size_t lds = 0;
-    const void* fn[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    const void* fn[9] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 
 constexpr int kMaxVariants = 12;

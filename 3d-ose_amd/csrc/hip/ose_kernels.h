@@ -10,7 +10,8 @@
 //   2. k_zsym              forward-z, * |k|/N, inverse-z   (in place on S)
 //   3. k_strided<SIGN=+1>  inverse-y       (in place on S)
 //   4. k_xpass<MODE>       inverse-x of S, recursion on physical W, taper,
-//                          accumulator, receivers, forward-x of W_{j+1}
+//                          accumulator (on about half of the orders, two
+//                          terms at a time), receivers, forward-x of W_{j+1}
 // Passes 1-3 run in place on X_j = FFT_x W_j; pass 4 reads W_{j-1} in
 // physical form and overwrites it with W_{j+1} (two physical W buffers
 // alternate), so each order needs one inverse and one forward x transform.
@@ -324,7 +325,20 @@ enum XMode : int {
     XM_FWD = 3,      // Xout = FFT_x(A)              (state upload)
     XM_INV = 4,      // A = IFFT_x(Xprev) / nx       (transform utilities)
     XM_LASTA = 5,    // as LAST without the forward transform (the boundary network acts on u first)
+    XM_SKIP = 6,     // W' = W_{j-1} - 2i L W_j;  out W', X'  (the accumulator is not touched)
+    XM_ORDER0N = 7,  // W1 = -i L W0;  out W1, X1             (the accumulator is not touched)
+    XM_MIDF = 8,     // as MID, the first update of the step's accumulator (no A read)
 };
+
+// Deferred accumulation.  Pass j (j >= 1) holds two terms of the series,
+// W_{j-1} (read) and W_{j+1} (written), so the accumulator need not be read
+// and written on every order: a touching pass adds the terms assigned to it
+// and catches up on the taper applications the skipped orders owed,
+//   A <- g^ea A + g^eb cp W_{j-1} + g^ec c1 W_{j+1}    (MIDF: no g^ea A term),
+// where the exponents count the taper applications each term has had by
+// order j in the reference's order of operations (g once per order j >= 1,
+// after adding W_{j+1}).  The host plans which orders touch A; the sum is the
+// reference's exactly, up to FP32 rounding of the powers of g.
 
 // The recursion runs on physical W: pass 4 of order j reads W_{j-1} and
 // writes W_{j+1} over it (Win == Wout except where noted), so only the
@@ -350,7 +364,9 @@ struct XArgs {
     int ny;
     int nlines;             // ny * nz
     float c0;               // ORDER0: J0
-    float c1;               // ORDER0: 2 J1; MID/LAST: 2 J_{j+1}
+    float c1;               // ORDER0: 2 J1; MID/LAST: coefficient of W_{j+1} (0: not added here)
+    float cp;               // MID/LAST: coefficient of W_{j-1} (0: not added here)
+    int ea, eb, ec;         // MID/LAST: powers of g (0..3) on A, W_{j-1}, W_{j+1}
     float inv_nx;
     // receivers (LAST): Im(u) on the plane z = rec_z over the interior
     float* traces;          // [shot][nt_cap][rec_ny][rec_nx]
@@ -463,6 +479,14 @@ __device__ __forceinline__ void x_forward(float2* __restrict__ u, bool okl, int 
     }
 }
 
+// g^e for a wave-uniform e in [0, 3] (products of g, as the reference
+// applies g once per order); the host keeps every exponent in that range
+__device__ __forceinline__ float gpow(float g, int e)
+{
+    const float g1 = (e & 1) ? g : 1.f;
+    return (e & 2) ? g1 * (g * g) : g1;
+}
+
 template <int N1, int N2, int P, int MODE, int SPLIT, int TWB>
 __global__ __launch_bounds__((XCfg<N1, N2, P, SPLIT>::THREADS))
 void k_xpass(XArgs a)
@@ -526,22 +550,33 @@ void k_xpass(XArgs a)
         for (int n1 = 0; n1 < N1; ++n1) {
             const int x = N2 * n1 + t;
             const float vr = a.vr[vbase + x];
-            const float2 wp = win[x];
             const float2 lw = make_float2(vr * u[n1].x, vr * u[n1].y);
             float2 wn;
-            if (MODE == XM_ORDER0) {
+            if (MODE == XM_ORDER0 || MODE == XM_ORDER0N) {
                 // W1 = -i L W0
                 wn = make_float2(lw.y, -lw.x);
-                acc[x] = make_float2(a.c0 * wp.x + a.c1 * wn.x, a.c0 * wp.y + a.c1 * wn.y);
+                if (MODE == XM_ORDER0) {
+                    const float2 wp = win[x];
+                    acc[x] = make_float2(a.c0 * wp.x + a.c1 * wn.x, a.c0 * wp.y + a.c1 * wn.y);
+                }
             } else {
                 // W_{j+1} = W_{j-1} - 2i L W_j
+                const float2 wp = win[x];
                 wn = make_float2(wp.x + 2.f * lw.y, wp.y - 2.f * lw.x);
-                const float g = gyz * a.gx[x];
-                const float2 ao = acc[x];
-                const float2 an = make_float2(g * (ao.x + a.c1 * wn.x), g * (ao.y + a.c1 * wn.y));
-                if (MODE == XM_MID) acc[x] = an;
-                if (MODE == XM_LAST || MODE == XM_LASTA)
-                    wn = an;   // the new state u; its x transform is X_0 of the next step
+                if (MODE != XM_SKIP) {
+                    const float g = gyz * a.gx[x];
+                    const float gp = gpow(g, a.eb) * a.cp, gn = gpow(g, a.ec) * a.c1;
+                    float2 an = make_float2(gp * wp.x + gn * wn.x, gp * wp.y + gn * wn.y);
+                    if (MODE != XM_MIDF) {
+                        const float2 ao = acc[x];
+                        const float ga = gpow(g, a.ea);
+                        an.x += ga * ao.x;
+                        an.y += ga * ao.y;
+                    }
+                    if (MODE == XM_MID || MODE == XM_MIDF) acc[x] = an;
+                    if (MODE == XM_LAST || MODE == XM_LASTA)
+                        wn = an;   // the new state u; its x transform is X_0 of the next step
+                }
             }
             u[n1] = wn;
         }

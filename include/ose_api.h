@@ -178,8 +178,11 @@ int ose_fft3d(ose_ctx* ctx, int nx, int ny, int nz, const float* in, float* out,
  *            so ose_pass_stats can report per-pass durations.              */
 int ose_set_options(ose_ctx* ctx, int use_graph, int profile);
 /* Per-pass statistics since the last reset: pass 0 forward-y, 1 z-symbol,
- * 2 inverse-y, 3 x-epilogue (middle orders), 4 x-epilogue (first order),
- * 5 x-epilogue (last order), 6 injection.  avg_ms is the mean HIP-event
+ * 2 inverse-y, 3 x-epilogue (middle orders that update the accumulator),
+ * 4 x-epilogue (first order), 5 x-epilogue (last order), 6 injection,
+ * 7 x-epilogue (middle orders that leave the accumulator alone; the step
+ * updates it on about half of the orders, adding two terms each time).
+ * avg_ms is the mean HIP-event
  * duration of one launch, bytes the mean algorithmic HBM bytes of one
  * launch (launches of several models differ in size), so bytes / avg_ms
  * is the pass's aggregate rate.                                             */
@@ -262,6 +265,15 @@ int ose_bnet_train(ose_bnet* net, int iters, float lr, unsigned long long seed, 
 int ose_supported_extent(int n);
 /* Nonzero if extent n is compiled.                                         */
 int ose_extent_is_supported(int n);
+/* The step's accumulator schedule for K orders (host only, no device).  The
+ * series sum A = sum_m c_m g^(e_m) W_m of wavefield_extro_ori2
+ * (ose_3d_lib.cu:741-859: add_bessel_3d then apply_ABC_Xt on every order
+ * j >= 1) is formed by only some orders' x passes, each adding the terms it
+ * holds (W_{j-1} and W_{j+1}).  plan[6*j ..] = {touch, adds W_{j-1} (W_0 at
+ * j = 0), adds W_{j+1} (W_1 at j = 0), ea, eb, ec}: the pass sets
+ * A <- g^ea A + g^eb c_{j-1} W_{j-1} + g^ec c_{j+1} W_{j+1} (ea < 0: no
+ * earlier A).  K in [2, 64].                                               */
+int ose_accumulation_plan(int K, int* plan);
 
 #ifdef __cplusplus
 }
