@@ -196,7 +196,8 @@ using namespace ose;
 namespace {
 
 enum PassKind {
-    PK_YF = 0, PK_Z = 1, PK_YI = 2, PK_XMID = 3, PK_X0 = 4, PK_XLAST = 5, PK_INJ = 6, PK_XSKIP = 7, PK_N = 8
+    PK_YF = 0, PK_Z = 1, PK_YI = 2, PK_XMID = 3, PK_X0 = 4, PK_XLAST = 5, PK_INJ = 6, PK_XSKIP = 7, PK_XMIDF = 8,
+    PK_N = 9
 };
 
 // Which orders of a step touch the accumulator (see the deferred
@@ -306,6 +307,7 @@ struct Model {
     const FftEntry *ex = nullptr, *ey = nullptr, *ez = nullptr;
     // chosen launch shapes (index into the entry's variants)
     int sel_yf = 0, sel_z = 0, sel_yi = 0, sel_x = 0;
+    int sel_xs = 0;                // x-pass shape of the orders that skip the accumulator
     bool tuned = false;
     long ncell = 0;
     DevBuf vr, g, k2, tw;          // g: gx|gy|gz, k2: kx2|ky2|kz2, tw: twx|twy|twz
@@ -744,13 +746,14 @@ int enqueue_model_step(ose_ctx* c, Model& m, int* d_it)
                 if (m.learned_on) bytes -= 8.0 * cells;   // LASTA writes no X_0
             } else if (pl.touch) {
                 mode = pl.ea >= 0 ? XM_MID : XM_MIDF;
-                kind = PK_XMID;
+                kind = pl.ea >= 0 ? PK_XMID : PK_XMIDF;
             } else {
                 mode = XM_SKIP;
                 kind = PK_XSKIP;
             }
         }
-        if ((rc = launch_x(c, kind, m.ex->xk[m.sel_x], mode, a, ny * nz, nshot, bytes))) return rc;
+        const XKern& xk = m.ex->xk[mode == XM_SKIP ? m.sel_xs : m.sel_x];
+        if ((rc = launch_x(c, kind, xk, mode, a, ny * nz, nshot, bytes))) return rc;
     }
     if (m.learned_on) {
         // boundary network on the pad of the new state u (in W[0]), then X0 = FFT_x(u)
@@ -843,12 +846,12 @@ int autotune_model(ose_ctx* c, Model& m)
                                                          m.ncell, m.d_twy, nullptr, 0.0); }, reps, &t);
         if (!rc && t < best) best = t, m.sel_yi = v;
     }
-    // x pass: the step's middle orders either update the accumulator (MID)
-    // or leave it alone (SKIP); weigh each variant's two times by how often
-    // the step runs each mode
-    int n_mid = 0, n_skip = 0;
-    for (int j = 1; j + 1 < m.d.K; ++j) (m.acc_plan[j].touch ? n_mid : n_skip)++;
-    if (n_mid + n_skip == 0) n_mid = 1;
+    // x pass: the step's middle orders either update the accumulator (MID;
+    // its shape also runs the first and last orders) or leave it alone
+    // (SKIP), each with its own fastest shape
+    bool any_skip = false;
+    for (int j = 1; j + 1 < m.d.K; ++j) any_skip = any_skip || !m.acc_plan[j].touch;
+    float best_s = 1e30f;
     best = 1e30f;
     for (int v = 0; v < m.ex->nx && !rc; ++v) {
         XArgs a = base_xargs(m, nshot);
@@ -857,14 +860,15 @@ int autotune_model(ose_ctx* c, Model& m)
         a.Win = a.Wout = m.W[1];
         a.Xout = m.X;
         a.c1 = 0.5f;
-        float ts = 0.f;
         rc = time_variant(c, [&] { return launch_x(c, -1, m.ex->xk[v], XM_MID, a, ny * nz, nshot, 0.0); }, reps, &t);
-        if (!rc && n_skip)
+        if (!rc && t < best) best = t, m.sel_x = v;
+        if (!rc && any_skip) {
             rc = time_variant(c, [&] { return launch_x(c, -1, m.ex->xk[v], XM_SKIP, a, ny * nz, nshot, 0.0); },
-                              reps, &ts);
-        const float tw = n_mid * t + n_skip * ts;
-        if (!rc && tw < best) best = tw, m.sel_x = v;
+                              reps, &t);
+            if (!rc && t < best_s) best_s = t, m.sel_xs = v;
+        }
     }
+    if (!any_skip) m.sel_xs = m.sel_x;
     c->profile = prof;
     if (!rc) m.tuned = true;
     return rc;
@@ -1786,7 +1790,7 @@ int ose_set_tuning(ose_ctx* c, int model_id, const int* shape)
     m.sel_yf = yf;
     m.sel_z = z;
     m.sel_yi = yi;
-    m.sel_x = x;
+    m.sel_x = m.sel_xs = x;
     m.tuned = true;
     drop_graph(c);
     return OSE_OK;

@@ -233,9 +233,9 @@ class Context:
         return out
 
     def pass_stats(self):
-        """{pass: (avg_ms, launches, bytes_per_launch)} for the 8 pass kinds."""
+        """{pass: (avg_ms, launches, bytes_per_launch)} for the 9 pass kinds."""
         out = {}
-        names = ["fwd_y", "z_symbol", "inv_y", "x_epilogue", "x_first", "x_last", "inject", "x_recur"]
+        names = ["fwd_y", "z_symbol", "inv_y", "x_epilogue", "x_first", "x_last", "inject", "x_recur", "x_acc_first"]
         for i, n in enumerate(names):
             ms, cnt, by = c_double(), c_long(), c_double()
             self._check(self._L.ose_pass_stats(self._h, i, ctypes.byref(ms), ctypes.byref(cnt), ctypes.byref(by)))

@@ -290,7 +290,7 @@ class Simulation:
         self._hip.ose_reset_stats(self.ctx)
 
     def pass_stats(self):
-        names = ["fwd_y", "z_symbol", "inv_y", "x_epilogue", "x_first", "x_last", "inject", "x_recur"]
+        names = ["fwd_y", "z_symbol", "inv_y", "x_epilogue", "x_first", "x_last", "inject", "x_recur", "x_acc_first"]
         out = {}
         for i, n in enumerate(names):
             ms, cnt, by = ctypes.c_double(), ctypes.c_long(), ctypes.c_double()

@@ -38,8 +38,8 @@ PASS_BYTES_DOC = {
     "fwd_y": "16 B/cell (read X_j, write it in place)",
     "z_symbol": "16 B/cell (read, write in place)",
     "inv_y": "16 B/cell (read, write in place: S)",
-    "x_epilogue": ("52 B/cell (read S, W_{j-1}, v/R, A; write X_{j+1}, W_{j+1}, A); 44 on the order that "
-                   "starts the accumulator (no A read); bytes_per_launch is the mean over a step's launches"),
+    "x_epilogue": "52 B/cell (read S, W_{j-1}, v/R, A; write X_{j+1}, W_{j+1}, A)",
+    "x_acc_first": "44 B/cell (read S, W_{j-1}, v/R; write X_{j+1}, W_{j+1}, A: the order that starts A)",
     "x_recur": "36 B/cell (read S, W_{j-1}, v/R; write X_{j+1}, W_{j+1}; accumulator not touched)",
     "x_first": "28 B/cell (read S, v/R; write X_1, W_1); 44 where it starts the accumulator (+W_0 read, A write)",
     "x_last": "44 B/cell (read S, W_{K-2}, v/R, A; write X_0, u)",

@@ -181,7 +181,8 @@ int ose_set_options(ose_ctx* ctx, int use_graph, int profile);
  * 2 inverse-y, 3 x-epilogue (middle orders that update the accumulator),
  * 4 x-epilogue (first order), 5 x-epilogue (last order), 6 injection,
  * 7 x-epilogue (middle orders that leave the accumulator alone; the step
- * updates it on about half of the orders, adding two terms each time).
+ * updates it on about half of the orders, adding two terms each time),
+ * 8 x-epilogue (the middle order that starts the accumulator: no A read).
  * avg_ms is the mean HIP-event
  * duration of one launch, bytes the mean algorithmic HBM bytes of one
  * launch (launches of several models differ in size), so bytes / avg_ms
@@ -197,11 +198,14 @@ int ose_set_autotune(ose_ctx* ctx, int on);
 /* shape[4] = {T forward-y, T z-symbol, T inverse-y, x-pass code} of a
  * model; x-pass code = P + 1000 * b where b = 1 marks the variant with a
  * scheduling barrier before the twiddle multiplies; a negative value marks
- * the half-LDS (split real/imaginary) exchange.                           */
+ * the half-LDS (split real/imaginary) exchange.  The x code is the shape of
+ * the orders that update the accumulator; the autotuner times the orders
+ * that leave it alone separately and may run them in another shape.       */
 int ose_get_tuning(ose_ctx* ctx, int model_id, int* shape);
 /* Pin a model's launch shapes (same encoding as ose_get_tuning; the
- * strided entries must be compiled for the y and z extents).  Skips the
- * autotuner for that model.  No counterpart in the reference.             */
+ * strided entries must be compiled for the y and z extents; the x code
+ * applies to every x pass).  Skips the autotuner for that model.  No
+ * counterpart in the reference.                                           */
 int ose_set_tuning(ose_ctx* ctx, int model_id, const int* shape);
 /* The compiled launch shapes of a model: strided codes usable on both the y
  * and z passes, and x-pass codes.  On entry *nstrided, *nxs are the array
