@@ -408,6 +408,10 @@ struct SnapStream {
 struct ose_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // second lane of the step (enqueue_step) and its fork/join events
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    int nstreams = 2;
     std::vector<std::unique_ptr<Model>> models;
     std::vector<Shot> shots;
     DevBuf it_buf;
@@ -416,8 +420,11 @@ struct ose_ctx {
     bool use_graph = true;
     bool profile = false;
     bool autotune = true;
-    hipGraphExec_t gexec = nullptr;
-    hipGraph_t graph = nullptr;
+    // the step's graphs: [0] the whole step, or with two lanes [0] lane 0's
+    // head, [1] lane 0's body, [2] lane 1, [3] the tail
+    hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipGraph_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
+    int ngraph = 0;
     std::string err;
     // profiling
     std::vector<hipEvent_t> ev_pool;
@@ -651,28 +658,36 @@ XArgs base_xargs(const Model& m, int nshot)
     return a;
 }
 
-// one OSE step of every shot of model m (the body of wavefield_extro_ori2
-// plus injection and receiver gather)
-int enqueue_model_step(ose_ctx* c, Model& m, int* d_it)
+// Injection and the K operator applications of one OSE step (the body of
+// wavefield_extro_ori2) for the shots [s0, s0 + nshot) of model m: every
+// per-shot buffer is offset to shot s0 and the launches cover nshot shots.
+// Only the launches numbered [p0, p1) are enqueued: 0 is the injection,
+// 1 + 4j .. 4 + 4j are order j's forward-y, z, inverse-y and x passes.
+int enqueue_model_orders(ose_ctx* c, Model& m, int* d_it, int s0, int nshot, int p0 = 0, int p1 = 1 << 30)
 {
-    const int nshot = (int)m.shots.size();
-    if (nshot == 0) return OSE_OK;
+    if (nshot <= 0) return OSE_OK;
+    auto in = [&](int p) { return p >= p0 && p < p1; };
     const int nx = m.d.nxb, ny = m.d.nyb, nz = m.d.nzb;
     const long cells = m.ncell * nshot;
+    const long so = m.ncell * s0;                // field offset of shot s0
+    float2* const X = m.X + so;
+    float2* const W[2] = {m.W[0] + so, m.W[1] + so};
+    float2* const A = m.A + so;
     int rc;
     // source injection into X0 (x-spectral) and W_0 (physical)
-    if (m.prof_wy > 0) {
+    if (!in(0)) {
+    } else if (m.prof_wy > 0) {
         ProfArgs a{};
-        a.X0 = m.X;
-        a.W0 = m.W[0];
+        a.X0 = X;
+        a.W0 = W[0];
         a.shot_stride = m.ncell;
         a.nx = nx;
         a.ny = ny;
-        a.desc = m.d_pdesc;
-        a.PX = m.d_PX;
-        a.pv = m.d_pv;
-        a.s = m.d_s;
-        a.h = m.d_h;
+        a.desc = m.d_pdesc + s0;
+        a.PX = m.d_PX + (long)s0 * nx;
+        a.pv = m.d_pv + (long)s0 * 3 * PROF_W;
+        a.s = m.d_s + (long)s0 * m.nt_cap;
+        a.h = m.d_h + (long)s0 * m.nt_cap;
         a.nt = m.nt_cap;
         a.it = d_it;
         a.wy_max = m.prof_wy;
@@ -681,14 +696,14 @@ int enqueue_model_step(ose_ctx* c, Model& m, int* d_it)
             return rc;
     } else {
         InjectArgs a{};
-        a.X0 = m.X;
-        a.W0 = m.W[0];
+        a.X0 = X;
+        a.W0 = W[0];
         a.shot_stride = m.ncell;
         a.nx = nx;
-        a.src_line = m.d_src_line;
-        a.src_x = m.d_src_x;
-        a.s = m.d_s;
-        a.h = m.d_h;
+        a.src_line = m.d_src_line + s0;
+        a.src_x = m.d_src_x + s0;
+        a.s = m.d_s + (long)s0 * m.nt_cap;
+        a.h = m.d_h + (long)s0 * m.nt_cap;
         a.nt = m.nt_cap;
         a.it = d_it;
         a.tw = m.d_twx;
@@ -699,32 +714,33 @@ int enqueue_model_step(ose_ctx* c, Model& m, int* d_it)
     const int K = m.d.K;
     for (int j = 0; j < K; ++j) {
         // passes 1-3 in place on X_j
-        if ((rc = launch_strided(c, PK_YF, m.ey->yf[m.sel_yf], m.X, m.X, nx, ny, nz, false, nshot,
-                                 m.ncell, m.d_twy, nullptr, 16.0 * cells)))
+        if (in(1 + 4 * j) && (rc = launch_strided(c, PK_YF, m.ey->yf[m.sel_yf], X, X, nx, ny, nz, false, nshot,
+                                                  m.ncell, m.d_twy, nullptr, 16.0 * cells)))
             return rc;
-        if ((rc = launch_strided(c, PK_Z, m.ez->zs[m.sel_z], m.X, m.X, nx, ny, nz, true, nshot,
-                                 m.ncell, m.d_twz, &m, 16.0 * cells)))
+        if (in(2 + 4 * j) && (rc = launch_strided(c, PK_Z, m.ez->zs[m.sel_z], X, X, nx, ny, nz, true, nshot,
+                                                  m.ncell, m.d_twz, &m, 16.0 * cells)))
             return rc;
-        if ((rc = launch_strided(c, PK_YI, m.ey->yi[m.sel_yi], m.X, m.X, nx, ny, nz, false, nshot,
-                                 m.ncell, m.d_twy, nullptr, 16.0 * cells)))
+        if (in(3 + 4 * j) && (rc = launch_strided(c, PK_YI, m.ey->yi[m.sel_yi], X, X, nx, ny, nz, false, nshot,
+                                                  m.ncell, m.d_twy, nullptr, 16.0 * cells)))
             return rc;
+        if (!in(4 + 4 * j)) continue;
         XArgs a = base_xargs(m, nshot);
-        a.S = m.X;
-        a.A = m.A;
-        a.Xout = m.X;
+        a.S = X;
+        a.A = A;
+        a.Xout = X;
         const AccPlan& pl = m.acc_plan[j];
         int mode, kind;
         double bytes = x_pass_bytes(j, K, pl.touch, pl.ea >= 0) * cells;
         if (j == 0) {
             mode = pl.touch ? XM_ORDER0 : XM_ORDER0N;
             kind = PK_X0;
-            a.Win = m.W[0];
-            a.Wout = m.W[1];
+            a.Win = W[0];
+            a.Wout = W[1];
             a.c0 = pl.add_prev ? (float)m.coef[0] : 0.f;
             a.c1 = pl.add_next ? (float)m.coef[1] : 0.f;
         } else {
             // W_{j-1} lives in W[(j-1)%2] and W_{j+1} replaces it
-            a.Win = a.Wout = m.W[(j - 1) % 2];
+            a.Win = a.Wout = W[(j - 1) % 2];
             a.c1 = pl.add_next ? (float)m.coef[j + 1] : 0.f;
             a.cp = pl.add_prev ? (float)m.coef[j - 1] : 0.f;
             a.ea = pl.ea;
@@ -733,9 +749,9 @@ int enqueue_model_step(ose_ctx* c, Model& m, int* d_it)
             if (j == K - 1) {
                 mode = m.learned_on ? XM_LASTA : XM_LAST;
                 kind = PK_XLAST;
-                a.Wout = m.W[0];    // the new state u
-                a.traces = m.d_traces;
+                a.Wout = W[0];      // the new state u
                 a.trace_shot_stride = (long)m.nt_cap * m.rec_ny * m.rec_nx;
+                a.traces = m.d_traces ? m.d_traces + s0 * a.trace_shot_stride : nullptr;
                 a.rec_z = m.d.bu + m.d.rec_z;
                 a.rec_y0 = m.d.bf;
                 a.rec_ny = m.rec_ny;
@@ -755,6 +771,18 @@ int enqueue_model_step(ose_ctx* c, Model& m, int* d_it)
         const XKern& xk = m.ex->xk[mode == XM_SKIP ? m.sel_xs : m.sel_x];
         if ((rc = launch_x(c, kind, xk, mode, a, ny * nz, nshot, bytes))) return rc;
     }
+    return OSE_OK;
+}
+
+// The rest of model m's step, over all of its shots: the boundary network
+// (learned pad) and the listed-receiver gather.
+int enqueue_model_tail(ose_ctx* c, Model& m, int* d_it)
+{
+    const int nshot = (int)m.shots.size();
+    if (nshot == 0) return OSE_OK;
+    const int ny = m.d.nyb, nz = m.d.nzb;
+    const long cells = m.ncell * nshot;
+    int rc;
     if (m.learned_on) {
         // boundary network on the pad of the new state u (in W[0]), then X0 = FFT_x(u)
         if ((rc = bnet_forward(c->stream, m.bg, m.d_wbf, m.d_bias, m.W[0], m.d_vr, m.ncell, nshot, m.bb, m.guard,
@@ -787,12 +815,126 @@ int enqueue_model_step(ose_ctx* c, Model& m, int* d_it)
     return OSE_OK;
 }
 
-int enqueue_step(ose_ctx* c)
+// The step's lanes.  With two streams (the default outside per-pass
+// profiling) and shots on two or more models, the shots are dealt to two
+// lanes -- a model of two or more shots in halves, single-shot models
+// whole, each item to the lane with fewer padded cells so far -- that run
+// side by side on the main and the side stream, forked and joined by
+// events.  The lanes touch disjoint shots, so they need no other ordering.
+// The boundary network, the receiver gather and the step counter (the
+// tail) follow the join on the main stream.  Measured on MI355X: 16 shots
+// of 600^2 x 480 plus four smaller models step 4 % faster in two lanes (the
+// small models' launches no longer run alone), while the shots of a single
+// model gain nothing (768^3 x 8 shots: 0.6 %, within noise; the 784^3 runs
+// lose 2-3 %), so one model's shots stay in one lane.  lane[1] is left
+// empty where there is nothing to split.
+struct LaneItem {
+    Model* m;
+    int s0, n;
+    double cells;
+};
+
+void plan_lanes(ose_ctx* c, std::vector<LaneItem> lane[2])
+{
+    lane[0].clear();
+    lane[1].clear();
+    std::vector<LaneItem> items;
+    int models = 0;
+    for (auto& m : c->models) models += !m->shots.empty();
+    const bool split = c->nstreams > 1 && !c->profile && c->side && models >= 2;
+    for (auto& m : c->models) {
+        const int n = (int)m->shots.size();
+        if (n == 0) continue;
+        if (split && n >= 2) {
+            const int h = n - n / 2;
+            items.push_back({m.get(), 0, h, (double)m->ncell * h});
+            items.push_back({m.get(), h, n - h, (double)m->ncell * (n - h)});
+        } else {
+            items.push_back({m.get(), 0, n, (double)m->ncell * n});
+        }
+    }
+    if (!split || items.size() < 2) {
+        lane[0] = items;
+        return;
+    }
+    std::stable_sort(items.begin(), items.end(), [](const LaneItem& a, const LaneItem& b) { return a.cells > b.cells; });
+    double load[2] = {0.0, 0.0};
+    for (const LaneItem& it : items) {
+        const int l = load[1] < load[0] ? 1 : 0;
+        lane[l].push_back(it);
+        load[l] += it.cells;
+    }
+}
+
+// part 0: the whole lane; 1: the head of its first item (injection,
+// forward-y and z of order 0); 2: the rest.  The second lane starts after
+// the first lane's head, so the two lanes run two passes apart and one's
+// z pass meets the other's x or y passes instead of another z pass.
+constexpr int kLaneHead = 3;
+
+int enqueue_lane(ose_ctx* c, const std::vector<LaneItem>& items, int part)
+{
+    int rc;
+    for (size_t i = 0; i < items.size(); ++i) {
+        const LaneItem& it = items[i];
+        int p0 = 0, p1 = 1 << 30;
+        if (part == 1) {
+            if (i) break;
+            p1 = kLaneHead;
+        } else if (part == 2 && i == 0) {
+            p0 = kLaneHead;
+        }
+        if ((rc = enqueue_model_orders(c, *it.m, c->d_it, it.s0, it.n, p0, p1))) return rc;
+    }
+    return OSE_OK;
+}
+
+int enqueue_tail(ose_ctx* c)
 {
     int rc;
     for (auto& m : c->models)
-        if ((rc = enqueue_model_step(c, *m, c->d_it))) return rc;
+        if ((rc = enqueue_model_tail(c, *m, c->d_it))) return rc;
     return launch(c, -1, 0.0, (const void*)&k_advance, dim3(1), dim3(64), 0, &c->d_it);
+}
+
+// fork the side stream off the main one / join it back
+int fork_side(ose_ctx* c)
+{
+    HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    return OSE_OK;
+}
+
+int join_side(ose_ctx* c)
+{
+    HIPCHK(c, hipEventRecord(c->ev_join, c->side));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    return OSE_OK;
+}
+
+// one step, launched eagerly
+int enqueue_step(ose_ctx* c)
+{
+    std::vector<LaneItem> lane[2];
+    plan_lanes(c, lane);
+    int rc;
+    if (lane[1].empty()) {
+        if ((rc = enqueue_lane(c, lane[0], 0))) return rc;
+        return enqueue_tail(c);
+    }
+    if ((rc = enqueue_lane(c, lane[0], 1))) return rc;
+    if ((rc = fork_side(c))) return rc;
+    rc = enqueue_lane(c, lane[0], 2);
+    if (!rc) {
+        hipStream_t main = c->stream;
+        c->stream = c->side;   // launch() enqueues on c->stream
+        rc = enqueue_lane(c, lane[1], 0);
+        c->stream = main;
+    }
+    const int jr = join_side(c);   // joined even after a failed launch
+    if (rc) return rc;
+    if (jr) return jr;
+    return enqueue_tail(c);
 }
 
 // Launch-shape autotuner: time every compiled variant of each pass on the
@@ -1040,34 +1182,84 @@ int finalize(ose_ctx* c)
     return OSE_OK;
 }
 
-int build_graph(ose_ctx* c)
+void drop_graph(ose_ctx* c);
+
+// capture what body enqueues on the main stream into an executable graph
+int capture_graph(ose_ctx* c, const std::function<int()>& body, hipGraph_t* g, hipGraphExec_t* x)
 {
-    if (c->gexec) return OSE_OK;
+    *g = nullptr;
+    *x = nullptr;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_step(c);
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    int rc = body();
+    hipGraph_t gg = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &gg);
     if (rc) {
-        if (g) (void)hipGraphDestroy(g);
+        if (gg) (void)hipGraphDestroy(gg);
         return rc;
     }
     if (e != hipSuccess) return c->fail(OSE_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
-    e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(x, gg, nullptr, nullptr, 0);
     if (e != hipSuccess) {
-        (void)hipGraphDestroy(g);
-        c->gexec = nullptr;
+        (void)hipGraphDestroy(gg);
+        *x = nullptr;
         return c->fail(OSE_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
     }
-    c->graph = g;
+    *g = gg;
+    return OSE_OK;
+}
+
+// The step as graphs: one graph, or with two lanes the first lane's head
+// and body, the second lane and the tail.  A graph's kernels run one after
+// another on the stream it is launched on, so the lanes are separate graphs
+// launched on two streams.
+int build_graph(ose_ctx* c)
+{
+    if (c->ngraph) return OSE_OK;
+    std::vector<LaneItem> lane[2];
+    plan_lanes(c, lane);
+    int rc;
+    if (lane[1].empty()) {
+        rc = capture_graph(c, [&] { int r = enqueue_lane(c, lane[0], 0); return r ? r : enqueue_tail(c); },
+                           &c->graph[0], &c->gexec[0]);
+        if (!rc) c->ngraph = 1;
+    } else {
+        rc = capture_graph(c, [&] { return enqueue_lane(c, lane[0], 1); }, &c->graph[0], &c->gexec[0]);
+        if (!rc) rc = capture_graph(c, [&] { return enqueue_lane(c, lane[0], 2); }, &c->graph[1], &c->gexec[1]);
+        if (!rc) rc = capture_graph(c, [&] { return enqueue_lane(c, lane[1], 0); }, &c->graph[2], &c->gexec[2]);
+        if (!rc) rc = capture_graph(c, [&] { return enqueue_tail(c); }, &c->graph[3], &c->gexec[3]);
+        if (!rc) c->ngraph = 4;
+    }
+    if (rc) drop_graph(c);
+    return rc;
+}
+
+// one step from the built graphs
+int launch_graph_step(ose_ctx* c)
+{
+    if (c->ngraph == 1) {
+        HIPCHK(c, hipGraphLaunch(c->gexec[0], c->stream));
+        return OSE_OK;
+    }
+    int rc;
+    HIPCHK(c, hipGraphLaunch(c->gexec[0], c->stream));
+    if ((rc = fork_side(c))) return rc;
+    HIPCHK(c, hipGraphLaunch(c->gexec[1], c->stream));
+    const hipError_t e = hipGraphLaunch(c->gexec[2], c->side);
+    if ((rc = join_side(c))) return rc;
+    if (e != hipSuccess) return c->fail(OSE_EDEVICE, "hipGraphLaunch: %s", hipGetErrorString(e));
+    HIPCHK(c, hipGraphLaunch(c->gexec[3], c->stream));
     return OSE_OK;
 }
 
 void drop_graph(ose_ctx* c)
 {
-    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-    if (c->graph) (void)hipGraphDestroy(c->graph);
-    c->gexec = nullptr;
-    c->graph = nullptr;
+    for (int i = 0; i < 4; ++i) {
+        if (c->gexec[i]) (void)hipGraphExecDestroy(c->gexec[i]);
+        if (c->graph[i]) (void)hipGraphDestroy(c->graph[i]);
+        c->gexec[i] = nullptr;
+        c->graph[i] = nullptr;
+    }
+    c->ngraph = 0;
 }
 
 Shot* get_shot(ose_ctx* c, int id)
@@ -1321,7 +1513,12 @@ int ose_create(ose_ctx** out, int device)
         return OSE_ENOMEM;
     }
     c->d_it = (int*)c->it_buf.p;
-    if (hipMemset(c->d_it, 0, sizeof(int)) != hipSuccess) {
+    if (hipMemset(c->d_it, 0, sizeof(int)) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+        if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+        if (c->side) (void)hipStreamDestroy(c->side);
         (void)hipStreamDestroy(c->stream);
         delete c;
         return OSE_EDEVICE;
@@ -1335,6 +1532,7 @@ int ose_destroy(ose_ctx* c)
     if (!c) return OSE_EINVAL;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->side) (void)hipStreamSynchronize(c->side);
     (void)snap_stop(c);
     drop_graph(c);
     for (auto& p : c->pending) {
@@ -1344,8 +1542,25 @@ int ose_destroy(ose_ctx* c)
     for (auto e : c->ev_pool)
         if (e) (void)hipEventDestroy(e);
     c->models.clear();
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+    return OSE_OK;
+}
+
+int ose_set_streams(ose_ctx* c, int n)
+{
+    if (!c) return OSE_EINVAL;
+    if (n != 1 && n != 2) return c->fail(OSE_EINVAL, "ose_set_streams: n must be 1 or 2");
+    if (n != c->nstreams) {
+        int rc;
+        if ((rc = set_device(c))) return rc;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->nstreams = n;
+        drop_graph(c);
+    }
     return OSE_OK;
 }
 
@@ -1586,8 +1801,8 @@ int ose_run(ose_ctx* c, int nsteps)
     const bool snaps = c->snap && c->snap->every > 0;
     if (snaps && (rc = snap_start(c))) return rc;
     for (int s = 0; s < nsteps; ++s) {
-        if (c->use_graph && !c->profile && c->gexec) {
-            HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+        if (c->use_graph && !c->profile && c->ngraph) {
+            if ((rc = launch_graph_step(c))) return rc;
         } else if ((rc = enqueue_step(c))) {
             return rc;
         }

@@ -110,6 +110,7 @@ def hip():
         _proto(L, "ose_supported_extent", [c_int])
         _proto(L, "ose_extent_is_supported", [c_int])
         _proto(L, "ose_accumulation_plan", [c_int, ip])
+        _proto(L, "ose_set_streams", [vp, c_int])
         _hip = L
     return _hip
 
@@ -140,5 +141,5 @@ def hip_symbols():
         "ose_copy_probe", "ose_pass_probe", "ose_boundary_net_size", "ose_set_boundary_net",
         "ose_boundary_status", "ose_bnet_create", "ose_bnet_destroy", "ose_bnet_error",
         "ose_bnet_set_params", "ose_bnet_get_params", "ose_bnet_forward", "ose_bnet_gradient",
-        "ose_bnet_add_sample", "ose_bnet_train", "ose_accumulation_plan",
+        "ose_bnet_add_sample", "ose_bnet_train", "ose_accumulation_plan", "ose_set_streams",
     ]

@@ -176,6 +176,10 @@ class Context:
     def set_options(self, use_graph=True, profile=False):
         self._check(self._L.ose_set_options(self._h, int(use_graph), int(profile)))
 
+    def set_streams(self, n):
+        """2: the step's shots run as two concurrent lanes; 1: one stream."""
+        self._check(self._L.ose_set_streams(self._h, int(n)))
+
     # -- stepping -----------------------------------------------------
     def run(self, nsteps):
         self._check(self._L.ose_run(self._h, int(nsteps)))

@@ -136,7 +136,14 @@ def tuning(sim, nmodels):
     return out
 
 
-def learned_leg(params, steps, warmup, plain_value, weights):
+def set_streams(sim, n):
+    from ose_amd import _lib
+    rc = _lib.hip().ose_set_streams(sim.ctx, n)
+    if rc:
+        raise RuntimeError(f"ose_set_streams({n}) failed: {rc}")
+
+
+def learned_leg(params, steps, warmup, plain_value, weights, streams):
     """The same workload with an 8-cell pad: plain thin taper, then the
     learned pad (network in the step, replacing the taper)."""
     from ose_amd.host import Simulation
@@ -144,6 +151,7 @@ def learned_leg(params, steps, warmup, plain_value, weights):
     for name, extra in (("thin_taper", ""), ("learned", f"bnet={weights}")):
         sim = Simulation(f"{params} boundary=8 nt={warmup + steps + 1} {extra}")
         sim.setup()
+        set_streams(sim, streams)
         info = sim.info()
         sim.timed(warmup)
         ms = sim.timed(steps)
@@ -183,6 +191,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the 8-cell learned-pad leg")
+    ap.add_argument("--streams", type=int, default=2, choices=(1, 2),
+                    help="2: the step's shots run as two concurrent lanes (default); 1: one stream")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run one plain device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -199,6 +209,7 @@ def main():
     sim = Simulation(f"{params} nt={nt} {args.extra}")
     t0 = time.time()
     sim.setup()
+    set_streams(sim, args.streams)
     info = sim.info()
     print(f"[bench] set-up {time.time() - t0:.1f} s: {json.dumps(info)}", file=sys.stderr, flush=True)
 
@@ -242,7 +253,7 @@ def main():
     learned = None
     if not args.no_learned and args.config in ("c4", "c1"):
         from ose_amd.boundary import WEIGHTS
-        learned = learned_leg(params + " " + args.extra, args.steps, args.warmup, value, WEIGHTS)
+        learned = learned_leg(params + " " + args.extra, args.steps, args.warmup, value, WEIGHTS, args.streams)
     out = {
         "metric": METRIC,
         "value": value,
@@ -268,6 +279,7 @@ def main():
             "padded_cells_per_step": info["padded_cells"],
             "padded_rate": info["padded_cells"] * args.steps / (ms * 1e-3) / 1e9,
             "ms_per_step_profiled": ms_prof / prof_steps,
+            "streams": args.streams,
             "launch_shapes": tune,
         },
         "learned": learned,

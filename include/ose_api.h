@@ -177,6 +177,14 @@ int ose_fft3d(ose_ctx* ctx, int nx, int ny, int nz, const float* in, float* out,
  * profile:   record HIP events around every pass kernel (eager launches)
  *            so ose_pass_stats can report per-pass durations.              */
 int ose_set_options(ose_ctx* ctx, int use_graph, int profile);
+/* Streams a step runs on: 2 (default) deals the shots of a context with two
+ * or more models to two lanes that run side by side (a model of two or more
+ * shots in halves, the second lane two launches behind the first); 1 runs
+ * every launch in order on one stream.  A single model's shots always run
+ * in one lane, and per-pass profiling (ose_set_options profile = 1) always
+ * uses one stream.  No counterpart in the reference (its shots run one
+ * after another, ose_3d.cu:242).                                          */
+int ose_set_streams(ose_ctx* ctx, int n);
 /* Per-pass statistics since the last reset: pass 0 forward-y, 1 z-symbol,
  * 2 inverse-y, 3 x-epilogue (middle orders that update the accumulator),
  * 4 x-epilogue (first order), 5 x-epilogue (last order), 6 injection,
