@@ -44,9 +44,17 @@ def strided_variants(n1, n2):
     nt = max(n1, n2)
     out = []
     for t in STRIDED_TS:
-        if t * nt > 896:      # larger workgroups spill the register DFTs (T = 32 needs N1, N2 <= 28)
-            continue
         full = 8 * n1 * n2 * (t + 1)
+        if t * nt > 1024:
+            continue
+        if t * nt > 896:
+            # 1024-thread workgroups (N1 or N2 of 29..32) spill 20-30 VGPRs of
+            # the register DFTs; compiled, split form only, for the long
+            # extents, where 256-byte rows halve the z pass's page visits
+            # per byte; the autotuner keeps them only where they are faster
+            if n1 * n2 >= 600 and full // 2 <= LDS_MAX:
+                out.append((t, 1))
+            continue
         if full <= LDS_MAX:
             out.append((t, 0))
         if full > LDS_MAX // 4 and full // 2 <= LDS_MAX:
