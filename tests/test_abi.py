@@ -45,6 +45,7 @@ def test_host_only_entry_points():
     assert L.ose_supported_extent(141) == 144
     assert L.ose_add_model(None, None, None, None) == _lib.OSE_EINVAL
     assert L.ose_run(None, 1) == _lib.OSE_EINVAL
+    assert L.ose_set_streams(None, 2) == _lib.OSE_EINVAL
 
 
 def test_create_without_device_reports_edevice():

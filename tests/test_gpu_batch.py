@@ -19,10 +19,12 @@ def _smooth_field(shape, seed):
     return np.fft.ifftn(F)
 
 
-@pytest.mark.parametrize("K", [8, 9, 12, 13])
+@pytest.mark.parametrize("K", [2, 3, 4, 5, 6, 7, 8, 9, 12, 13])
 def test_one_step_odd_and_even_order(K):
     """The last order writes the new state over W_{K-2} (K even) or into
-    the other physical buffer (K odd)."""
+    the other physical buffer (K odd); K = 2..9 also cover every shape of
+    the deferred accumulator schedule (which orders update it, from the
+    first order on for K = 2 and 5, and where it starts)."""
     import ose_amd
     print(f"one step K={K}", flush=True)
     case = Case((36, 20, 36), (6, 6, 6, 6, 6, 6), layered(36, 20, 36, seed=7), K=K)

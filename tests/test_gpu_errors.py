@@ -82,7 +82,10 @@ def test_shot_and_state_errors():
         re[1, 2, 3] = np.nan
         assert _err(lambda: c.set_wavefield(sid, re, im))[0] == OSE_ENUMERIC
         assert _err(lambda: c.fft3d(np.zeros((16, 16, 17), np.complex64)))[0] != 0
-        tr = c.traces(sid, 0, 5)
+        assert _err(lambda: c.set_streams(3))[0] == OSE_EINVAL
+        c.set_streams(1)
+        c.run(1)
+        tr = c.traces(sid, 0, 6)
         assert np.all(np.isfinite(tr))
 
 
