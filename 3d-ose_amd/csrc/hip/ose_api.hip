@@ -1,10 +1,11 @@
 // ose_api.hip -- C-ABI implementation of the HIP backend (include/ose_api.h).
 //
-// A context owns one HIP stream and, per model, the velocity (as v/R), the
-// taper profiles, the symbol tables and the twiddle tables, and per shot
-// batch the four complex fields of the fused step:
-//     X0, X1  x-spectral Chebyshev terms (ping-pong slots)
-//     S       spectrum scratch of passes 1-3
+// A context owns a main and a side HIP stream and, per model, the velocity
+// (as v/R), the taper profiles, the symbol tables and the twiddle tables,
+// and per shot batch the five complex fields of the fused step:
+//     X       x-spectral Chebyshev term (forward-y in place)
+//     S       its spectrum (z pass out of place from X, inverse-y in place)
+//     W0, W1  physical Chebyshev terms (W0 holds the state u between steps)
 //     A       accumulator (physical space)
 // Shots of one model are batched into one launch per pass (blockIdx.z /
 // blockIdx.y = shot), so a model's step is 4K + 1 launches and one step of
@@ -319,9 +320,11 @@ struct Model {
     std::vector<int> shots;        // context shot ids
     int nt_cap = 0;
     DevBuf fields, traces, src, wav;
-    // X: x-spectral work field (passes 1-3 in place), W[0] the physical state
-    // u = W_0 and with W[1] the alternating physical W_j, A the accumulator
-    float2 *X = nullptr, *W[2] = {nullptr, nullptr}, *A = nullptr;
+    // X: x-spectral work field (forward-y in place), S the spectrum (the z
+    // pass writes it out of place from X; inverse-y in place on it), W[0]
+    // the physical state u = W_0 and with W[1] the alternating physical W_j,
+    // A the accumulator
+    float2 *X = nullptr, *S = nullptr, *W[2] = {nullptr, nullptr}, *A = nullptr;
     float* d_traces = nullptr;
     long* d_src_line = nullptr;
     int* d_src_x = nullptr;
@@ -671,6 +674,7 @@ int enqueue_model_orders(ose_ctx* c, Model& m, int* d_it, int s0, int nshot, int
     const long cells = m.ncell * nshot;
     const long so = m.ncell * s0;                // field offset of shot s0
     float2* const X = m.X + so;
+    float2* const S = m.S + so;
     float2* const W[2] = {m.W[0] + so, m.W[1] + so};
     float2* const A = m.A + so;
     int rc;
@@ -713,19 +717,21 @@ int enqueue_model_orders(ose_ctx* c, Model& m, int* d_it, int s0, int nshot, int
     }
     const int K = m.d.K;
     for (int j = 0; j < K; ++j) {
-        // passes 1-3 in place on X_j
+        // passes 1-3: forward-y in place on X_j, the z pass from X into S
+        // (out of place: a z-pencil workgroup that writes other pages than
+        // it reads runs faster), inverse-y in place on S
         if (in(1 + 4 * j) && (rc = launch_strided(c, PK_YF, m.ey->yf[m.sel_yf], X, X, nx, ny, nz, false, nshot,
                                                   m.ncell, m.d_twy, nullptr, 16.0 * cells)))
             return rc;
-        if (in(2 + 4 * j) && (rc = launch_strided(c, PK_Z, m.ez->zs[m.sel_z], X, X, nx, ny, nz, true, nshot,
+        if (in(2 + 4 * j) && (rc = launch_strided(c, PK_Z, m.ez->zs[m.sel_z], X, S, nx, ny, nz, true, nshot,
                                                   m.ncell, m.d_twz, &m, 16.0 * cells)))
             return rc;
-        if (in(3 + 4 * j) && (rc = launch_strided(c, PK_YI, m.ey->yi[m.sel_yi], X, X, nx, ny, nz, false, nshot,
+        if (in(3 + 4 * j) && (rc = launch_strided(c, PK_YI, m.ey->yi[m.sel_yi], S, S, nx, ny, nz, false, nshot,
                                                   m.ncell, m.d_twy, nullptr, 16.0 * cells)))
             return rc;
         if (!in(4 + 4 * j)) continue;
         XArgs a = base_xargs(m, nshot);
-        a.S = X;
+        a.S = S;
         a.A = A;
         a.Xout = X;
         const AccPlan& pl = m.acc_plan[j];
@@ -978,13 +984,13 @@ int autotune_model(ose_ctx* c, Model& m)
     }
     best = 1e30f;
     for (int v = 0; v < m.ez->ns && !rc; ++v) {
-        rc = time_variant(c, [&] { return launch_strided(c, -1, m.ez->zs[v], m.X, m.X, nx, ny, nz, true, nshot,
+        rc = time_variant(c, [&] { return launch_strided(c, -1, m.ez->zs[v], m.X, m.S, nx, ny, nz, true, nshot,
                                                          m.ncell, m.d_twz, &m, 0.0); }, reps, &t);
         if (!rc && t < best) best = t, m.sel_z = v;
     }
     best = 1e30f;
     for (int v = 0; v < m.ey->ns && !rc; ++v) {
-        rc = time_variant(c, [&] { return launch_strided(c, -1, m.ey->yi[v], m.X, m.X, nx, ny, nz, false, nshot,
+        rc = time_variant(c, [&] { return launch_strided(c, -1, m.ey->yi[v], m.S, m.S, nx, ny, nz, false, nshot,
                                                          m.ncell, m.d_twy, nullptr, 0.0); }, reps, &t);
         if (!rc && t < best) best = t, m.sel_yi = v;
     }
@@ -997,7 +1003,7 @@ int autotune_model(ose_ctx* c, Model& m)
     best = 1e30f;
     for (int v = 0; v < m.ex->nx && !rc; ++v) {
         XArgs a = base_xargs(m, nshot);
-        a.S = m.X;
+        a.S = m.S;
         a.A = m.A;
         a.Win = a.Wout = m.W[1];
         a.Xout = m.X;
@@ -1087,13 +1093,14 @@ int finalize(ose_ctx* c)
             HIPCHK(c, hipMemsetAsync(m.rtr.p, 0, sizeof(float) * (size_t)rsamples, c->stream));
         }
         const size_t fbytes = sizeof(float2) * (size_t)m.ncell * nshot;
-        if ((rc = dev_alloc(c, m.fields, 4 * fbytes))) return rc;
+        if ((rc = dev_alloc(c, m.fields, 5 * fbytes))) return rc;
         char* base = (char*)m.fields.p;
         m.X = (float2*)base;
         m.W[0] = (float2*)(base + fbytes);
         m.W[1] = (float2*)(base + 2 * fbytes);
         m.A = (float2*)(base + 3 * fbytes);
-        HIPCHK(c, hipMemsetAsync(m.fields.p, 0, 4 * fbytes, c->stream));
+        m.S = (float2*)(base + 4 * fbytes);
+        HIPCHK(c, hipMemsetAsync(m.fields.p, 0, 5 * fbytes, c->stream));
         const size_t tbytes = sizeof(float) * (size_t)m.nt_cap * m.rec_ny * m.rec_nx * nshot;
         if ((rc = dev_alloc(c, m.traces, tbytes))) return rc;
         m.d_traces = (float*)m.traces.p;
