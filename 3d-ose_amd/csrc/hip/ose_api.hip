@@ -717,16 +717,15 @@ int enqueue_model_orders(ose_ctx* c, Model& m, int* d_it, int s0, int nshot, int
     }
     const int K = m.d.K;
     for (int j = 0; j < K; ++j) {
-        // passes 1-3: forward-y in place on X_j, the z pass from X into S
-        // (out of place: a z-pencil workgroup that writes other pages than
-        // it reads runs faster), inverse-y in place on S
-        if (in(1 + 4 * j) && (rc = launch_strided(c, PK_YF, m.ey->yf[m.sel_yf], X, X, nx, ny, nz, false, nshot,
+        // passes 1-3 out of place, alternating between X and S: forward-y
+        // X -> S, the z pass S -> X, inverse-y X -> S
+        if (in(1 + 4 * j) && (rc = launch_strided(c, PK_YF, m.ey->yf[m.sel_yf], X, S, nx, ny, nz, false, nshot,
                                                   m.ncell, m.d_twy, nullptr, 16.0 * cells)))
             return rc;
-        if (in(2 + 4 * j) && (rc = launch_strided(c, PK_Z, m.ez->zs[m.sel_z], X, S, nx, ny, nz, true, nshot,
+        if (in(2 + 4 * j) && (rc = launch_strided(c, PK_Z, m.ez->zs[m.sel_z], S, X, nx, ny, nz, true, nshot,
                                                   m.ncell, m.d_twz, &m, 16.0 * cells)))
             return rc;
-        if (in(3 + 4 * j) && (rc = launch_strided(c, PK_YI, m.ey->yi[m.sel_yi], S, S, nx, ny, nz, false, nshot,
+        if (in(3 + 4 * j) && (rc = launch_strided(c, PK_YI, m.ey->yi[m.sel_yi], X, S, nx, ny, nz, false, nshot,
                                                   m.ncell, m.d_twy, nullptr, 16.0 * cells)))
             return rc;
         if (!in(4 + 4 * j)) continue;
@@ -978,19 +977,19 @@ int autotune_model(ose_ctx* c, Model& m)
     const int reps = std::max(1, std::min(5, (int)(2e8 / ((double)m.ncell * nshot))));
     best = 1e30f;
     for (int v = 0; v < m.ey->ns && !rc; ++v) {
-        rc = time_variant(c, [&] { return launch_strided(c, -1, m.ey->yf[v], m.X, m.X, nx, ny, nz, false, nshot,
+        rc = time_variant(c, [&] { return launch_strided(c, -1, m.ey->yf[v], m.X, m.S, nx, ny, nz, false, nshot,
                                                          m.ncell, m.d_twy, nullptr, 0.0); }, reps, &t);
         if (!rc && t < best) best = t, m.sel_yf = v;
     }
     best = 1e30f;
     for (int v = 0; v < m.ez->ns && !rc; ++v) {
-        rc = time_variant(c, [&] { return launch_strided(c, -1, m.ez->zs[v], m.X, m.S, nx, ny, nz, true, nshot,
+        rc = time_variant(c, [&] { return launch_strided(c, -1, m.ez->zs[v], m.S, m.X, nx, ny, nz, true, nshot,
                                                          m.ncell, m.d_twz, &m, 0.0); }, reps, &t);
         if (!rc && t < best) best = t, m.sel_z = v;
     }
     best = 1e30f;
     for (int v = 0; v < m.ey->ns && !rc; ++v) {
-        rc = time_variant(c, [&] { return launch_strided(c, -1, m.ey->yi[v], m.S, m.S, nx, ny, nz, false, nshot,
+        rc = time_variant(c, [&] { return launch_strided(c, -1, m.ey->yi[v], m.X, m.S, nx, ny, nz, false, nshot,
                                                          m.ncell, m.d_twy, nullptr, 0.0); }, reps, &t);
         if (!rc && t < best) best = t, m.sel_yi = v;
     }
