@@ -3,8 +3,9 @@
 // A context owns a main and a side HIP stream and, per model, the velocity
 // (as v/R), the taper profiles, the symbol tables and the twiddle tables,
 // and per shot batch the five complex fields of the fused step:
-//     X       x-spectral Chebyshev term (forward-y in place)
-//     S       its spectrum (z pass out of place from X, inverse-y in place)
+//     X, S    x-spectral work fields: passes 1-3 run X -> S -> X -> S and
+//             pass 4 reads the spectrum from S (S is X, in place, for
+//             batches under 1 GiB per field)
 //     W0, W1  physical Chebyshev terms (W0 holds the state u between steps)
 //     A       accumulator (physical space)
 // Shots of one model are batched into one launch per pass (blockIdx.z /
@@ -320,10 +321,9 @@ struct Model {
     std::vector<int> shots;        // context shot ids
     int nt_cap = 0;
     DevBuf fields, traces, src, wav;
-    // X: x-spectral work field (forward-y in place), S the spectrum (the z
-    // pass writes it out of place from X; inverse-y in place on it), W[0]
-    // the physical state u = W_0 and with W[1] the alternating physical W_j,
-    // A the accumulator
+    // X, S: x-spectral work fields (passes 1-3 run X -> S -> X -> S; S == X
+    // for small batches, which run them in place), W[0] the physical state
+    // u = W_0 and with W[1] the alternating physical W_j, A the accumulator
     float2 *X = nullptr, *S = nullptr, *W[2] = {nullptr, nullptr}, *A = nullptr;
     float* d_traces = nullptr;
     long* d_src_line = nullptr;
@@ -718,7 +718,7 @@ int enqueue_model_orders(ose_ctx* c, Model& m, int* d_it, int s0, int nshot, int
     const int K = m.d.K;
     for (int j = 0; j < K; ++j) {
         // passes 1-3 out of place, alternating between X and S: forward-y
-        // X -> S, the z pass S -> X, inverse-y X -> S
+        // X -> S, the z pass S -> X, inverse-y X -> S (in place where S == X)
         if (in(1 + 4 * j) && (rc = launch_strided(c, PK_YF, m.ey->yf[m.sel_yf], X, S, nx, ny, nz, false, nshot,
                                                   m.ncell, m.d_twy, nullptr, 16.0 * cells)))
             return rc;
@@ -1091,15 +1091,21 @@ int finalize(ose_ctx* c)
             m.d_rtr = (float*)m.rtr.p;
             HIPCHK(c, hipMemsetAsync(m.rtr.p, 0, sizeof(float) * (size_t)rsamples, c->stream));
         }
+        // Batches of 1 GiB or more per field get the spectrum field S and
+        // run passes 1-3 out of place (faster in HBM); smaller ones keep S = X
+        // and run them in place, so a pass's working set (one field) can stay
+        // in the 256 MB Infinity Cache (a 272^3 field is 161 MB).
         const size_t fbytes = sizeof(float2) * (size_t)m.ncell * nshot;
-        if ((rc = dev_alloc(c, m.fields, 5 * fbytes))) return rc;
+        const bool oop = fbytes >= ((size_t)1 << 30);
+        const size_t nfield = oop ? 5 : 4;
+        if ((rc = dev_alloc(c, m.fields, nfield * fbytes))) return rc;
         char* base = (char*)m.fields.p;
         m.X = (float2*)base;
         m.W[0] = (float2*)(base + fbytes);
         m.W[1] = (float2*)(base + 2 * fbytes);
         m.A = (float2*)(base + 3 * fbytes);
-        m.S = (float2*)(base + 4 * fbytes);
-        HIPCHK(c, hipMemsetAsync(m.fields.p, 0, 5 * fbytes, c->stream));
+        m.S = oop ? (float2*)(base + 4 * fbytes) : m.X;
+        HIPCHK(c, hipMemsetAsync(m.fields.p, 0, nfield * fbytes, c->stream));
         const size_t tbytes = sizeof(float) * (size_t)m.nt_cap * m.rec_ny * m.rec_nx * nshot;
         if ((rc = dev_alloc(c, m.traces, tbytes))) return rc;
         m.d_traces = (float*)m.traces.p;
