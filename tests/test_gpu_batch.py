@@ -155,3 +155,29 @@ def test_two_stream_lanes_equal_one_stream(use_graph):
         assert np.abs(t1).max() > 0
         assert np.array_equal(t1, t2)
         assert np.array_equal(r1, r2) and np.array_equal(i1, i2)
+
+
+def test_out_of_place_batch_equals_in_place_shot():
+    """A batch of 1 GiB or more per field runs passes 1-3 out of place
+    through a second spectral field; a single shot of the same model stays
+    in place.  Same kernels and shapes: the shots agree bit for bit."""
+    import ose_amd
+    print("out-of-place batch (4 x 336^3) vs in-place shot", flush=True)
+    nt = 6
+    case = Case((324, 324, 324), (6, 6, 6, 6, 6, 6), layered(324, 324, 324, seed=31))
+    assert 4 * case.nxb * case.nyb * case.nzb * 8 >= 1 << 30 > case.nxb * case.nyb * case.nzb * 8
+    shots = _shots(case, [(162, 162, 2, 30.0, 2), (40, 200, 10, 25.0, 3), (300, 20, 30, 35.0, 2),
+                          (100, 100, 100, 28.0, 1)], nt)
+    out = {}
+    for n in (4, 1):
+        with ose_amd.Context(0) as c:
+            c.set_autotune(False)
+            m = case.add_to(c)
+            ids = [c.add_shot(m, (case.bl + sx, case.bf + sy, case.bu + sz), s, h)
+                   for (sx, sy, sz), s, h in shots[:n]]
+            c.run(nt)
+            out[n] = (c.traces(ids[0]), c.wavefield(ids[0]))
+    (t4, (r4, i4)), (t1, (r1, i1)) = out[4], out[1]
+    assert np.abs(r1).max() > 0
+    assert np.array_equal(t4, t1)
+    assert np.array_equal(r4, r1) and np.array_equal(i4, i1)
