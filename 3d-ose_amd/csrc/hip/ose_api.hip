@@ -309,7 +309,7 @@ struct Model {
     const FftEntry *ex = nullptr, *ey = nullptr, *ez = nullptr;
     // chosen launch shapes (index into the entry's variants)
     int sel_yf = 0, sel_z = 0, sel_yi = 0, sel_x = 0;
-    int sel_xs = 0;                // x-pass shape of the orders that skip the accumulator
+    int sel_mode[9] = {};          // x-pass shape per XMode the step runs (LASTA takes LAST's)
     bool tuned = false;
     long ncell = 0;
     DevBuf vr, g, k2, tw;          // g: gx|gy|gz, k2: kx2|ky2|kz2, tw: twx|twy|twz
@@ -773,7 +773,7 @@ int enqueue_model_orders(ose_ctx* c, Model& m, int* d_it, int s0, int nshot, int
                 kind = PK_XSKIP;
             }
         }
-        const XKern& xk = m.ex->xk[mode == XM_SKIP ? m.sel_xs : m.sel_x];
+        const XKern& xk = m.ex->xk[m.sel_mode[mode == XM_LASTA ? XM_LAST : mode]];
         if ((rc = launch_x(c, kind, xk, mode, a, ny * nz, nshot, bytes))) return rc;
     }
     return OSE_OK;
@@ -993,29 +993,40 @@ int autotune_model(ose_ctx* c, Model& m)
                                                          m.ncell, m.d_twy, nullptr, 0.0); }, reps, &t);
         if (!rc && t < best) best = t, m.sel_yi = v;
     }
-    // x pass: the step's middle orders either update the accumulator (MID;
-    // its shape also runs the first and last orders) or leave it alone
-    // (SKIP), each with its own fastest shape
-    bool any_skip = false;
-    for (int j = 1; j + 1 < m.d.K; ++j) any_skip = any_skip || !m.acc_plan[j].touch;
-    float best_s = 1e30f;
-    best = 1e30f;
+    // x pass: every mode the step runs gets its own fastest shape -- MID
+    // (middle orders that update the accumulator; also the shape of the
+    // transform utilities), SKIP (those that leave it alone), MIDF (the
+    // first update), the first order and the last order
+    bool any_skip = false, any_midf = false;
+    for (int j = 1; j + 1 < m.d.K; ++j) {
+        const AccPlan& pl = m.acc_plan[j];
+        any_skip = any_skip || !pl.touch;
+        any_midf = any_midf || (pl.touch && pl.ea < 0);
+    }
+    const int modes[5] = {XM_MID, XM_SKIP, XM_MIDF, m.acc_plan[0].touch ? XM_ORDER0 : XM_ORDER0N, XM_LAST};
+    const bool used[5] = {true, any_skip, any_midf, true, true};
+    float best_m[5] = {1e30f, 1e30f, 1e30f, 1e30f, 1e30f};
     for (int v = 0; v < m.ex->nx && !rc; ++v) {
+        // all on zero buffers; the last order records no receivers here
         XArgs a = base_xargs(m, nshot);
         a.S = m.S;
         a.A = m.A;
         a.Win = a.Wout = m.W[1];
         a.Xout = m.X;
         a.c1 = 0.5f;
-        rc = time_variant(c, [&] { return launch_x(c, -1, m.ex->xk[v], XM_MID, a, ny * nz, nshot, 0.0); }, reps, &t);
-        if (!rc && t < best) best = t, m.sel_x = v;
-        if (!rc && any_skip) {
-            rc = time_variant(c, [&] { return launch_x(c, -1, m.ex->xk[v], XM_SKIP, a, ny * nz, nshot, 0.0); },
+        a.traces = m.d_traces ? m.d_traces : reinterpret_cast<float*>(m.A);   // never written: rec_ny = 0
+        a.it = c->d_it;
+        a.rec_ny = 0;
+        for (int i = 0; i < 5 && !rc; ++i) {
+            if (!used[i]) continue;
+            rc = time_variant(c, [&] { return launch_x(c, -1, m.ex->xk[v], modes[i], a, ny * nz, nshot, 0.0); },
                               reps, &t);
-            if (!rc && t < best_s) best_s = t, m.sel_xs = v;
+            if (!rc && t < best_m[i]) best_m[i] = t, m.sel_mode[modes[i]] = v;
         }
     }
-    if (!any_skip) m.sel_xs = m.sel_x;
+    m.sel_x = m.sel_mode[XM_MID];
+    for (int i = 0; i < 5; ++i)
+        if (!used[i]) m.sel_mode[modes[i]] = m.sel_x;
     c->profile = prof;
     if (!rc) m.tuned = true;
     return rc;
@@ -2017,7 +2028,8 @@ int ose_set_tuning(ose_ctx* c, int model_id, const int* shape)
     m.sel_yf = yf;
     m.sel_z = z;
     m.sel_yi = yi;
-    m.sel_x = m.sel_xs = x;
+    m.sel_x = x;
+    for (int& k : m.sel_mode) k = x;
     m.tuned = true;
     drop_graph(c);
     return OSE_OK;
