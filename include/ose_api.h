@@ -207,8 +207,10 @@ int ose_set_autotune(ose_ctx* ctx, int on);
  * model; x-pass code = P + 1000 * b where b = 1 marks the variant with a
  * scheduling barrier before the twiddle multiplies; a negative value marks
  * the half-LDS (split real/imaginary) exchange.  The x code is the shape of
- * the orders that update the accumulator; the autotuner times the orders
- * that leave it alone separately and may run them in another shape.       */
+ * the middle orders that update the accumulator; the autotuner times the
+ * step's other x-pass kinds (orders that leave the accumulator alone, its
+ * first update, the first and the last order) separately and may run each
+ * in another shape.                                                       */
 int ose_get_tuning(ose_ctx* ctx, int model_id, int* shape);
 /* Pin a model's launch shapes (same encoding as ose_get_tuning; the
  * strided entries must be compiled for the y and z extents; the x code
