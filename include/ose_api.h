@@ -179,7 +179,8 @@ int ose_fft3d(ose_ctx* ctx, int nx, int ny, int nz, const float* in, float* out,
 int ose_set_options(ose_ctx* ctx, int use_graph, int profile);
 /* Streams a step runs on: 2 (default) deals the shots of a context with two
  * or more models to two lanes that run side by side (a model of two or more
- * shots in halves, the second lane two launches behind the first); 1 runs
+ * shots in halves, the second lane starting after the first lane's
+ * injection, forward-y and z pass); 1 runs
  * every launch in order on one stream.  A single model's shots always run
  * in one lane, and per-pass profiling (ose_set_options profile = 1) always
  * uses one stream.  No counterpart in the reference (its shots run one
