@@ -449,6 +449,9 @@ int oseh_sim_setup(oseh_sim* s)
     const int nt = s->geti("nt", 300);
     if ((rc = ose_create(&s->ctx, s->geti("device", 0)))) return s->fail(rc, "ose_create failed");
     ose_set_options(s->ctx, s->geti("graph", 1), 0);
+    // streams=1|2: one stream, or two concurrent lanes for several models
+    if ((rc = ose_set_streams(s->ctx, s->geti("streams", 2))))
+        return s->fail(rc, "streams must be 1 or 2");
     const int rec_z = s->geti("rec_z", 0);
     for (size_t mi = 0; mi < s->models.size(); ++mi) {
         BuiltModel& m = s->models[mi];

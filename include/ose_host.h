@@ -140,7 +140,8 @@ int oseh_gaussian_profile(double sigma, int center, int n, int* start, float* ou
  * Keys: nx ny nz dx dy dz dt nt fm amp delay boundary bl br bf bb bu bd
  *       model seed v0 v1 iface vfile sx sy sz nshot jsx jsy jsz srcgrid
  *       rec_z korder ktol taper alpha extent hilbert device graph
- *       max_shots (cap on shots per model) snap segy src_sigma
+ *       streams (1 or 2, ose_set_streams) max_shots (cap on shots per
+ *       model) snap segy src_sigma
  * Receivers: by default the whole interior plane rec_z.  ng=<n> records a
  * line (gxbeg + r*jgx, gybeg + r*jgy, gzbeg + r*jgz), r < n (the
  * reference's gxbeg..jgz, ng, ose_3d.cu:100-114; defaults jgx = 1,
