@@ -2035,6 +2035,47 @@ int ose_set_tuning(ose_ctx* c, int model_id, const int* shape)
     return OSE_OK;
 }
 
+// x-pass modes in the order of ose_get_tuning_modes' entries 3..7 (the first
+// order's entry covers ORDER0 and ORDER0N, the last order's LAST and LASTA)
+static const int kTuneModes[5] = {XM_MID, XM_SKIP, XM_MIDF, XM_ORDER0N, XM_LAST};
+
+int ose_get_tuning_modes(ose_ctx* c, int model_id, int* shape)
+{
+    if (!c || !shape) return OSE_EINVAL;
+    if (model_id < 0 || model_id >= (int)c->models.size()) return c->fail(OSE_EINVAL, "ose_get_tuning_modes: bad model");
+    const Model& m = *c->models[model_id];
+    int rc;
+    if ((rc = ose_get_tuning(c, model_id, shape))) return rc;
+    for (int i = 0; i < 5; ++i) shape[3 + i] = xcode(m.ex->xk[m.sel_mode[kTuneModes[i]]]);
+    // ORDER0 and ORDER0N share the first order's entry; the plan uses one of them
+    if (!m.acc_plan.empty() && m.acc_plan[0].touch) shape[6] = xcode(m.ex->xk[m.sel_mode[XM_ORDER0]]);
+    return OSE_OK;
+}
+
+int ose_set_tuning_modes(ose_ctx* c, int model_id, const int* shape)
+{
+    if (!c || !shape) return OSE_EINVAL;
+    if (model_id < 0 || model_id >= (int)c->models.size()) return c->fail(OSE_EINVAL, "ose_set_tuning_modes: bad model");
+    Model& m = *c->models[model_id];
+    int xs[5];
+    for (int i = 0; i < 5; ++i) {
+        xs[i] = -1;
+        for (int v = 0; v < m.ex->nx; ++v)
+            if (xcode(m.ex->xk[v]) == shape[3 + i]) xs[i] = v;
+        if (xs[i] < 0)
+            return c->fail(OSE_EINVAL, "ose_set_tuning_modes: x-pass shape %d is not compiled for extent %d",
+                           shape[3 + i], m.d.nxb);
+    }
+    const int base[4] = {shape[0], shape[1], shape[2], shape[3]};
+    int rc;
+    if ((rc = ose_set_tuning(c, model_id, base))) return rc;
+    for (int i = 0; i < 5; ++i) m.sel_mode[kTuneModes[i]] = xs[i];
+    m.sel_mode[XM_ORDER0] = xs[3];
+    m.sel_mode[XM_LASTA] = xs[4];
+    m.sel_x = xs[0];
+    return OSE_OK;
+}
+
 int ose_launch_shapes(ose_ctx* c, int model_id, int* strided, int* nstrided, int* xshapes, int* nxs)
 {
     if (!c || !nstrided || !nxs) return OSE_EINVAL;
@@ -2191,20 +2232,17 @@ int ose_set_wavefield(ose_ctx* c, int shot_id, const float* re, const float* im)
     return OSE_OK;
 }
 
-int ose_apply_operator(ose_ctx* c, int model_id, const float* in, float* out)
+}  // extern "C"
+
+namespace {
+
+// The operator application on two caller-provided device work fields of
+// m.ncell complex cells each (A: the field and the result, X: its spectrum).
+int apply_operator_on(ose_ctx* c, Model& m, const float* in, float* out, float2* A, float2* X)
 {
-    if (!c) return OSE_EINVAL;
-    if (model_id < 0 || model_id >= (int)c->models.size() || !in || !out)
-        return c->fail(OSE_EINVAL, "ose_apply_operator: bad model or null pointer");
     int rc;
-    if ((rc = set_device(c)) || (rc = ose_sync(c))) return rc;
-    Model& m = *c->models[model_id];
     const int nx = m.d.nxb, ny = m.d.nyb, nz = m.d.nzb;
-    DevBuf tmp;
     const size_t fb = sizeof(float2) * (size_t)m.ncell;
-    if ((rc = dev_alloc(c, tmp, 2 * fb))) return rc;
-    float2* A = (float2*)tmp.p;
-    float2* X = (float2*)((char*)tmp.p + fb);
     HIPCHK(c, hipMemcpy(A, in, fb, hipMemcpyHostToDevice));
     const bool prof = c->profile;
     c->profile = false;
@@ -2234,6 +2272,24 @@ int ose_apply_operator(ose_ctx* c, int model_id, const float* in, float* out)
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(out, A, fb, hipMemcpyDeviceToHost));
     return OSE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ose_apply_operator(ose_ctx* c, int model_id, const float* in, float* out)
+{
+    if (!c) return OSE_EINVAL;
+    if (model_id < 0 || model_id >= (int)c->models.size() || !in || !out)
+        return c->fail(OSE_EINVAL, "ose_apply_operator: bad model or null pointer");
+    int rc;
+    if ((rc = set_device(c)) || (rc = ose_sync(c))) return rc;
+    Model& m = *c->models[model_id];
+    DevBuf tmp;
+    const size_t fb = sizeof(float2) * (size_t)m.ncell;
+    if ((rc = dev_alloc(c, tmp, 2 * fb))) return rc;
+    return apply_operator_on(c, m, in, out, (float2*)tmp.p, (float2*)((char*)tmp.p + fb));
 }
 
 int ose_fft3d(ose_ctx* c, int nx, int ny, int nz, const float* in, float* out, int sign)

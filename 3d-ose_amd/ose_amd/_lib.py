@@ -94,6 +94,8 @@ def hip():
         _proto(L, "ose_set_autotune", [vp, c_int])
         _proto(L, "ose_get_tuning", [vp, c_int, ip])
         _proto(L, "ose_set_tuning", [vp, c_int, ip])
+        _proto(L, "ose_get_tuning_modes", [vp, c_int, ip])
+        _proto(L, "ose_set_tuning_modes", [vp, c_int, ip])
         _proto(L, "ose_launch_shapes", [vp, c_int, ip, ip, ip, ip])
         _proto(L, "ose_boundary_net_size", [])
         _proto(L, "ose_set_boundary_net", [vp, c_int, fp, ctypes.c_size_t, c_double, c_int])
@@ -137,7 +139,7 @@ def hip_symbols():
         "ose_set_wavefield", "ose_get_step", "ose_set_step", "ose_apply_operator",
         "ose_fft3d", "ose_set_options", "ose_pass_stats", "ose_reset_stats",
         "ose_supported_extent", "ose_extent_is_supported", "ose_set_autotune", "ose_get_tuning",
-        "ose_set_tuning", "ose_launch_shapes",
+        "ose_set_tuning", "ose_get_tuning_modes", "ose_set_tuning_modes", "ose_launch_shapes",
         "ose_copy_probe", "ose_pass_probe", "ose_boundary_net_size", "ose_set_boundary_net",
         "ose_boundary_status", "ose_bnet_create", "ose_bnet_destroy", "ose_bnet_error",
         "ose_bnet_set_params", "ose_bnet_get_params", "ose_bnet_forward", "ose_bnet_gradient",

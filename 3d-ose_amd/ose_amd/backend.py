@@ -152,6 +152,16 @@ class Context:
         arr = (c_int * 4)(t_fwd_y, t_z, t_inv_y, p_x)
         self._check(self._L.ose_set_tuning(self._h, model, arr))
 
+    def tuning_modes(self, model):
+        """All eight launch-shape codes of a model (ose_get_tuning_modes)."""
+        out = (c_int * 8)()
+        self._check(self._L.ose_get_tuning_modes(self._h, model, out))
+        return list(out)
+
+    def set_tuning_modes(self, model, shape):
+        arr = (c_int * 8)(*shape)
+        self._check(self._L.ose_set_tuning_modes(self._h, model, arr))
+
     def launch_shapes(self, model):
         """(strided codes usable on y and z, x-pass codes) compiled for a model."""
         ns, nx = c_int(16), c_int(16)

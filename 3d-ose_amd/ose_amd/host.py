@@ -323,6 +323,21 @@ class Simulation:
             raise OseError(rc, self._hip.ose_last_error(self.ctx).decode())
         return re, im
 
+    def set_wavefield(self, shot, re, im):
+        """Overwrite a shot's state (initial conditions) after setup()."""
+        re = np.ascontiguousarray(re, dtype=np.float32)
+        im = np.ascontiguousarray(im, dtype=np.float32)
+        rc = self._hip.ose_set_wavefield(self.ctx, shot, re.ctypes.data_as(_fp), im.ctypes.data_as(_fp))
+        if rc:
+            raise OseError(rc, self._hip.ose_last_error(self.ctx).decode())
+
+    def boundary_status(self, model):
+        a, t = ctypes.c_int(), ctypes.c_int()
+        rc = self._hip.ose_boundary_status(self.ctx, model, ctypes.byref(a), ctypes.byref(t))
+        if rc:
+            raise OseError(rc, self._hip.ose_last_error(self.ctx).decode())
+        return dict(active=bool(a.value), tripped=bool(t.value))
+
     # model/shot data (for the oracle)
     def nmodels(self):
         return self._L.oseh_sim_nmodels(self._h)

@@ -126,13 +126,16 @@ def traffic_from_profile(config, kernel):
 
 
 def tuning(sim, nmodels):
+    """Launch shapes per model: T of forward-y, z, inverse-y, then the x-pass
+    code of each x mode (middle orders that update the accumulator, those
+    that leave it alone, its first update, the first order, the last order)."""
     import ctypes
     from ose_amd import _lib
     out = []
     for m in range(nmodels):
-        t = (ctypes.c_int * 4)()
-        _lib.hip().ose_get_tuning(sim.ctx, m, t)
-        out.append(list(t))
+        t = (ctypes.c_int * 8)()
+        _lib.hip().ose_get_tuning_modes(sim.ctx, m, t)
+        out.append(dict(zip(("fwd_y", "z", "inv_y", "x_mid", "x_skip", "x_midf", "x_first", "x_last"), list(t))))
     return out
 
 
@@ -175,6 +178,87 @@ def learned_leg(params, steps, warmup, plain_value, weights, streams):
     }
 
 
+SNAP_CASES = (("f32", 3, False), ("bf16", 2, True), ("q16", 2, True), ("q8", 2, True))
+
+
+def snapshot_leg(params, steps, every, warmup=2):
+    """Cost of snapshot streaming on the running workload, wall clock around
+    the steps and the final drain: steps without snapshots; with snapshots
+    every `every` steps (ose_set_snapshots: encode on the device, pinned copy
+    on a second stream, a callback that only counts bytes) in several
+    encodings; and the synchronous dump of the reference's commented block
+    (ose_3d.cu:258-269: stop after the step, copy every shot's whole padded
+    state to the host, go on).  One JSON line per case, then a summary."""
+    import threading
+
+    import numpy as np
+    from ose_amd import _lib
+    from ose_amd.host import Simulation
+
+    nt = warmup + steps * (len(SNAP_CASES) + 3) + 8
+    sim = Simulation(f"{params} nt={nt}").setup()
+    hip = _lib.hip()
+    ctx = sim.ctx
+    info = sim.info()
+    shots = sim.nshots()
+    out = []
+    try:
+        sim.run(warmup)
+
+        def wall(fn):
+            hip.ose_sync(ctx)
+            t0 = time.perf_counter()
+            fn()
+            if hip.ose_sync(ctx):
+                raise RuntimeError(hip.ose_last_error(ctx).decode())
+            return time.perf_counter() - t0
+
+        def plain():
+            hip.ose_run(ctx, steps)
+
+        base = wall(plain)
+        out.append({"mode": "none", "ms_per_step": 1e3 * base / steps})
+        print(json.dumps(out[-1]), flush=True)
+        for fmt, lanes, interior in SNAP_CASES:
+            got = {"n": 0, "bytes": 0}
+            lock = threading.Lock()
+
+            def count(user, inf, data):
+                with lock:
+                    got["n"] += 1
+                    got["bytes"] += inf.contents.bytes
+
+            cb = _lib.SNAPSHOT_FN(count)
+            if hip.ose_set_snapshots(ctx, every, lanes, _lib.SNAP_FORMATS[fmt], int(interior), cb, None):
+                raise RuntimeError(hip.ose_last_error(ctx).decode())
+            t = wall(plain)
+            hip.ose_set_snapshots(ctx, 0, 0, 0, 0, _lib.SNAPSHOT_FN(), None)
+            out.append({"mode": "stream", "format": fmt, "lanes": "pq" if lanes == 3 else "q",
+                        "box": "interior" if interior else "padded", "every": every, "snapshots": got["n"],
+                        "gb_per_snapshot": got["bytes"] / max(got["n"], 1) / 1e9,
+                        "ms_per_step": 1e3 * t / steps, "overhead": t / base - 1.0})
+            print(json.dumps(out[-1]), flush=True)
+
+        n = max(int(np.prod(sim.model_dims(sim.shot(k)["model"])[9:12])) for k in range(shots))
+        re = np.empty(n, dtype=np.float32)
+        im = np.empty(n, dtype=np.float32)
+
+        def sync_dump():
+            for _ in range(steps // every):
+                hip.ose_run(ctx, every)
+                for k in range(shots):
+                    if hip.ose_get_wavefield(ctx, k, re.ctypes.data_as(_lib.fp), im.ctypes.data_as(_lib.fp)):
+                        raise RuntimeError(hip.ose_last_error(ctx).decode())
+
+        t = wall(sync_dump)
+        out.append({"mode": "sync_dump", "format": "f32", "lanes": "pq", "box": "padded", "every": every,
+                    "ms_per_step": 1e3 * t / steps, "overhead": t / base - 1.0})
+        print(json.dumps(out[-1]), flush=True)
+    finally:
+        sim.close()
+    return {"params": params, "models": info["models"], "shots": shots, "steps": steps, "runs": out}
+
+
 def main():
     if len(sys.argv) == 3 and sys.argv[1] == "--rocprof-summary":
         print(rocprof_summary(sys.argv[2]))
@@ -193,6 +277,9 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the 8-cell learned-pad leg")
     ap.add_argument("--streams", type=int, default=2, choices=(1, 2),
                     help="2: the step's shots run as two concurrent lanes (default); 1: one stream")
+    ap.add_argument("--snapshots", type=int, default=0, metavar="EVERY",
+                    help="instead of the throughput line: the cost of streaming snapshots every EVERY steps "
+                         "(one JSON line per encoding, then a summary)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run one plain device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -204,6 +291,9 @@ def main():
     from ose_amd.host import Simulation
 
     params, workload = CONFIGS[args.config]
+    if args.snapshots > 0:
+        print(json.dumps(snapshot_leg(f"{params} {args.extra}", args.steps, args.snapshots)))
+        return
     prof_steps = max(2, min(args.steps, 4))
     nt = args.warmup + args.steps + prof_steps
     sim = Simulation(f"{params} nt={nt} {args.extra}")

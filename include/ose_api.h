@@ -218,6 +218,15 @@ int ose_get_tuning(ose_ctx* ctx, int model_id, int* shape);
  * applies to every x pass).  Skips the autotuner for that model.  No
  * counterpart in the reference.                                           */
 int ose_set_tuning(ose_ctx* ctx, int model_id, const int* shape);
+/* Every launch shape of a model, one entry per pass kind: shape[8] =
+ * {T forward-y, T z-symbol, T inverse-y, x code of the middle orders that
+ * update the accumulator, of those that leave it alone, of its first
+ * update, of the first order, of the last order} (codes as ose_get_tuning).
+ * ose_set_tuning_modes pins all eight (skipping the autotuner for that
+ * model), so a configuration read back from one context reproduces its
+ * launches exactly in another.  No counterpart in the reference.         */
+int ose_get_tuning_modes(ose_ctx* ctx, int model_id, int* shape);
+int ose_set_tuning_modes(ose_ctx* ctx, int model_id, const int* shape);
 /* The compiled launch shapes of a model: strided codes usable on both the y
  * and z passes, and x-pass codes.  On entry *nstrided, *nxs are the array
  * capacities; on return, the counts.  No counterpart in the reference.   */
