@@ -2274,6 +2274,34 @@ int apply_operator_on(ose_ctx* c, Model& m, const float* in, float* out, float2*
     return OSE_OK;
 }
 
+// Scratch of ose_apply_operator_noalloc: its two work fields, each of at most
+// kNoAllocCells complex cells (64 MiB in all).  Every GPU the library runs on
+// holds its own copy until the process ends.  Host code never passes the
+// array's name to a kernel or a copy (on the host it is only a stand-in for
+// the symbol): it uses the device address hipGetSymbolAddress gives on each
+// device, taken once per device by noalloc_scratch.
+constexpr long kNoAllocCells = 1L << 22;
+__device__ __align__(256) float2 scratch[2 * kNoAllocCells];
+
+std::mutex g_scratch_mu;
+float2* g_scratch_ptr[64] = {};
+
+// the device address of this device's scratch (looked up once per device)
+int noalloc_scratch(ose_ctx* c, float2** ptr)
+{
+    if (c->device < 0 || c->device >= 64) return c->fail(OSE_EDEVICE, "no scratch for device %d", c->device);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[c->device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return c->fail(OSE_EDEVICE, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        g_scratch_ptr[c->device] = (float2*)p;
+    }
+    *ptr = g_scratch_ptr[c->device];
+    return OSE_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2290,6 +2318,22 @@ int ose_apply_operator(ose_ctx* c, int model_id, const float* in, float* out)
     const size_t fb = sizeof(float2) * (size_t)m.ncell;
     if ((rc = dev_alloc(c, tmp, 2 * fb))) return rc;
     return apply_operator_on(c, m, in, out, (float2*)tmp.p, (float2*)((char*)tmp.p + fb));
+}
+
+int ose_apply_operator_noalloc(ose_ctx* c, int model_id, const float* in, float* out)
+{
+    if (!c) return OSE_EINVAL;
+    if (model_id < 0 || model_id >= (int)c->models.size() || !in || !out)
+        return c->fail(OSE_EINVAL, "ose_apply_operator_noalloc: bad model or null pointer");
+    Model& m = *c->models[model_id];
+    if (m.ncell > kNoAllocCells)
+        return c->fail(OSE_EINVAL, "ose_apply_operator_noalloc: %ld cells exceed the scratch limit of %ld cells",
+                       m.ncell, kNoAllocCells);
+    int rc;
+    if ((rc = set_device(c)) || (rc = ose_sync(c))) return rc;
+    float2* s = nullptr;
+    if ((rc = noalloc_scratch(c, &s))) return rc;
+    return apply_operator_on(c, m, in, out, s, s + kNoAllocCells);
 }
 
 int ose_fft3d(ose_ctx* c, int nx, int ny, int nz, const float* in, float* out, int sign)

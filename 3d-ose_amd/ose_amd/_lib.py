@@ -85,6 +85,7 @@ def hip():
         _proto(L, "ose_get_step", [vp, ip])
         _proto(L, "ose_set_step", [vp, c_int])
         _proto(L, "ose_apply_operator", [vp, c_int, fp, fp])
+        _proto(L, "ose_apply_operator_noalloc", [vp, c_int, fp, fp])
         _proto(L, "ose_fft3d", [vp, c_int, c_int, c_int, fp, fp, c_int])
         _proto(L, "ose_set_options", [vp, c_int, c_int])
         _proto(L, "ose_pass_stats", [vp, c_int, dp, P(c_long), dp])
@@ -136,7 +137,7 @@ def hip_symbols():
         "ose_create", "ose_destroy", "ose_last_error", "ose_add_model", "ose_add_shot",
         "ose_run", "ose_sync", "ose_timed_run", "ose_get_traces", "ose_set_traces", "ose_set_receivers",
         "ose_receiver_count", "ose_set_source_profile", "ose_set_snapshots", "ose_snapshot_drain", "ose_get_wavefield",
-        "ose_set_wavefield", "ose_get_step", "ose_set_step", "ose_apply_operator",
+        "ose_set_wavefield", "ose_get_step", "ose_set_step", "ose_apply_operator", "ose_apply_operator_noalloc",
         "ose_fft3d", "ose_set_options", "ose_pass_stats", "ose_reset_stats",
         "ose_supported_extent", "ose_extent_is_supported", "ose_set_autotune", "ose_get_tuning",
         "ose_set_tuning", "ose_get_tuning_modes", "ose_set_tuning_modes", "ose_launch_shapes",

@@ -238,6 +238,14 @@ class Context:
                                                out.view(np.float32).ctypes.data_as(_fp)))
         return out
 
+    def apply_operator_noalloc(self, model, u):
+        """apply_operator through the entry point that allocates nothing."""
+        u = np.ascontiguousarray(u, dtype=np.complex64)
+        out = np.empty_like(u)
+        self._check(self._L.ose_apply_operator_noalloc(self._h, model, u.view(np.float32).ctypes.data_as(_fp),
+                                                       out.view(np.float32).ctypes.data_as(_fp)))
+        return out
+
     def fft3d(self, a, sign=-1):
         a = np.ascontiguousarray(a, dtype=np.complex64)
         out = np.empty_like(a)

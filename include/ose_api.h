@@ -168,6 +168,17 @@ int ose_set_step(ose_ctx* ctx, int it);
  * scale_kxkz_3D, INVERSE, scal_vr_3d).                                      */
 int ose_apply_operator(ose_ctx* ctx, int model_id, const float* in, float* out);
 
+/* ose_apply_operator without any device allocation: the two work fields
+ * live in a static device array of the library (64 MiB per GPU, held until
+ * the process ends), so models of at most 4194304 padded cells (e.g. 160^3)
+ * are accepted and larger ones fail with OSE_EINVAL.  Same inputs, outputs
+ * and results as ose_apply_operator, bit for bit.  Every call on one GPU
+ * shares that GPU's array, so calls that overlap on one GPU (several host
+ * threads, several contexts) race; serialise them.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc). */
+int ose_apply_operator_noalloc(ose_ctx* ctx, int model_id, const float* in, float* out);
+
 /* Unnormalised 3D DFT of an interleaved complex [nz][ny][nx] array with the
  * library's pass kernels; sign -1 forward, +1 inverse (cufftXtExecDescriptorC2C,
  * ose_3d_lib.cu:668).                                                       */
