@@ -32,6 +32,9 @@ SIZES = {
     864: (27, 32), 960: (30, 32), 1024: (32, 32),
 }
 
+# radices of the runtime-length path's register butterflies (ose_generic.h)
+GENERIC_RADICES = (2, 3, 4, 5, 7, 8, 11, 13, 16)
+
 STRIDED_TS = (8, 16, 32)
 LDS_MAX = 160 * 1024
 
@@ -139,7 +142,9 @@ def main():
     only = [int(a) for a in sys.argv[2:]]
     sizes = {n: SIZES[n] for n in (only or SIZES)}
     os.makedirs(out, exist_ok=True)
-    factors = sorted({f for (a, b) in SIZES.values() for f in (a, b)})
+    # register DFTs of the compiled splits and of the runtime-length path's
+    # radices (ose_generic.h)
+    factors = sorted({f for (a, b) in SIZES.values() for f in (a, b)} | set(GENERIC_RADICES))
     subprocess.check_call([sys.executable, os.path.join(HERE, "gen_dft.py"),
                            os.path.join(out, "dft_gen.h")] + [str(f) for f in factors])
     with open(os.path.join(out, "ose_sizes.inc"), "w") as f:

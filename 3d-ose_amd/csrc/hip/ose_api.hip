@@ -173,13 +173,20 @@ Registry& registry()
 }
 }  // namespace
 
-const FftEntry* find_entry(int N)
+const FftEntry* find_compiled(int N)
 {
     for (auto& e : registry().entries)
         if (e.N == N) return &e;
     return nullptr;
 }
 
+const FftEntry* find_entry(int N)
+{
+    const FftEntry* e = find_compiled(N);
+    return e ? e : generic_entry(N);
+}
+
+// smallest extent >= n with compiled kernels (the extent=smooth rounding)
 int next_supported(int n)
 {
     int best = 0;
@@ -423,6 +430,7 @@ struct ose_ctx {
     bool use_graph = true;
     bool profile = false;
     bool autotune = true;
+    bool generic_only = false;   // ose_set_fft_path(ctx, 1): runtime-length kernels on every axis
     // the step's graphs: [0] the whole step, or with two lanes [0] lane 0's
     // head, [1] lane 0's body, [2] lane 1, [3] the tail
     hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -489,7 +497,7 @@ int set_device(ose_ctx* c)
 // Angles in FP64, reduced mod n before the cos/sin.
 void roots(int n, std::vector<float2>& out)
 {
-    const FftEntry* e = find_entry(n);
+    const FftEntry* e = find_compiled(n);   // the runtime-length path reads block 0 only
     const int n1 = e ? e->N1 : n, n2 = e ? e->N2 : 1;
     out.resize(3 * (size_t)n);
     auto w = [n](long j) {
@@ -568,10 +576,12 @@ int launch_strided(ose_ctx* c, int kind, const StridedKern& k, const float2* in,
                    const Model* m, double bytes)
 {
     if (!k.fn || !in || !out || !tw) return c->fail(OSE_EINVAL, "strided pass launched with a missing buffer");
-    // the kernels address a thread's column with a 32-bit byte offset (col + t es) * 8, t < 32
-    if ((double)(k.T + 32.0 * (zaxis ? (double)nx * ny : (double)nx)) * sizeof(float2) >= 4294967296.0)
+    // the compiled kernels address a thread's column with a 32-bit byte
+    // offset (col + t es) * 8, t < 32 (the runtime-length kernels use 64 bits)
+    if (!k.gp && (double)(k.T + 32.0 * (zaxis ? (double)nx * ny : (double)nx)) * sizeof(float2) >= 4294967296.0)
         return c->fail(OSE_EUNSUPPORTED, "strided pass: plane of %d x %d cells too large", nx, ny);
     StridedArgs a{};
+    if (k.gp) a.g = *k.gp;
     a.in = in;
     a.out = out;
     a.ncol = nx;
@@ -599,6 +609,7 @@ int launch_x(ose_ctx* c, int kind, const XKern& k, int mode, XArgs& a, int nline
 {
     a.nlines = nlines;
     if (!a.twf) a.twf = a.tw;
+    if (k.gp) a.g = *k.gp;
     // every pointer the mode dereferences must be set before launching
     bool ok = a.tw != nullptr && k.fn[mode] != nullptr;
     switch (mode) {
@@ -1612,6 +1623,16 @@ int ose_accumulation_plan(int K, int* plan)
     return OSE_OK;
 }
 int ose_extent_is_supported(int n) { return find_entry(n) != nullptr; }
+int ose_extent_kind(int n) { return find_compiled(n) ? 1 : generic_entry(n) ? 2 : 0; }
+
+int ose_set_fft_path(ose_ctx* c, int path)
+{
+    if (!c) return OSE_EINVAL;
+    if (path != 0 && path != 1) return c->fail(OSE_EINVAL, "ose_set_fft_path: path must be 0 or 1");
+    if (!c->models.empty()) return c->fail(OSE_ESTATE, "ose_set_fft_path: models already added");
+    c->generic_only = path == 1;
+    return OSE_OK;
+}
 
 int ose_add_model(ose_ctx* c, const ose_model_desc* d, const float* v, int* model_id)
 {
@@ -1628,13 +1649,13 @@ int ose_add_model(ose_ctx* c, const ose_model_desc* d, const float* v, int* mode
         return c->fail(OSE_EINVAL, "ose_add_model: R, dx, dy, dz, dt must be positive");
     if (d->rec_z < 0 || d->rec_z >= d->nz) return c->fail(OSE_EINVAL, "ose_add_model: rec_z outside the interior");
     auto m = std::make_unique<Model>();
-    m->ex = find_entry(d->nxb);
-    m->ey = find_entry(d->nyb);
-    m->ez = find_entry(d->nzb);
+    auto entry = [c](int n) { return c->generic_only ? generic_entry(n) : find_entry(n); };
+    m->ex = entry(d->nxb);
+    m->ey = entry(d->nyb);
+    m->ez = entry(d->nzb);
     if (!m->ex || !m->ey || !m->ez)
-        return c->fail(OSE_EUNSUPPORTED, "ose_add_model: extent %dx%dx%d not compiled (next supported: %d %d %d)",
-                       d->nxb, d->nyb, d->nzb, next_supported(d->nxb), next_supported(d->nyb),
-                       next_supported(d->nzb));
+        return c->fail(OSE_EUNSUPPORTED, "ose_add_model: extent %dx%dx%d outside [2, %d]", d->nxb, d->nyb, d->nzb,
+                       GEN_NMAX);
     m->d = *d;
     m->coef.assign(d->coef, d->coef + d->K + 1);
     m->acc_plan = plan_accumulation(d->K);
@@ -2340,8 +2361,9 @@ int ose_fft3d(ose_ctx* c, int nx, int ny, int nz, const float* in, float* out, i
 {
     if (!c) return OSE_EINVAL;
     if (!in || !out || (sign != 1 && sign != -1)) return c->fail(OSE_EINVAL, "ose_fft3d: bad argument");
-    const FftEntry *ex = find_entry(nx), *ey = find_entry(ny), *ez = find_entry(nz);
-    if (!ex || !ey || !ez) return c->fail(OSE_EUNSUPPORTED, "ose_fft3d: extent not compiled");
+    auto entry = [c](int n) { return c->generic_only ? generic_entry(n) : find_entry(n); };
+    const FftEntry *ex = entry(nx), *ey = entry(ny), *ez = entry(nz);
+    if (!ex || !ey || !ez) return c->fail(OSE_EUNSUPPORTED, "ose_fft3d: extent outside [2, %d]", GEN_NMAX);
     int rc;
     if ((rc = set_device(c)) || (rc = ose_sync(c))) return rc;
     for (const FftEntry* e : {ex, ey, ez})

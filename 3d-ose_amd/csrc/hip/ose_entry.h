@@ -4,12 +4,15 @@
 
 namespace ose {
 
+struct GenPlan;
+
 // one compiled launch shape of a strided (y/z) pass: T x columns per
 // workgroup, split = 1 for the half-LDS (real, then imaginary) exchange
 struct StridedKern {
     int T = 0, split = 0, threads = 0;
     size_t lds = 0;
     const void* fn = nullptr;
+    const GenPlan* gp = nullptr;      // runtime-length kernels: their plan
 };
 
 // one compiled launch shape of the x pass: P lines per workgroup, a kernel
@@ -19,12 +22,13 @@ struct XKern {
     int P = 0, split = 0, twb = 0, threads = 0;
     size_t lds = 0;
     const void* fn[9] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    const GenPlan* gp = nullptr;      // runtime-length kernels: their plan
 };
 
 constexpr int kMaxVariants = 12;
 
 struct FftEntry {
-    int N = 0, N1 = 0, N2 = 0;
+    int N = 0, N1 = 0, N2 = 0;        // N1 = N2 = 0: the runtime-length path
     int ns = 0;                       // strided variants
     StridedKern yf[kMaxVariants];     // forward strided transform
     StridedKern yi[kMaxVariants];     // inverse strided transform
@@ -33,8 +37,15 @@ struct FftEntry {
     XKern xk[kMaxVariants];
 };
 
-// nullptr if N is not a compiled extent
+// the compiled kernels of extent N, else the runtime-length path (nullptr
+// outside [2, GEN_NMAX])
 const FftEntry* find_entry(int N);
+// the compiled kernels of extent N only
+const FftEntry* find_compiled(int N);
+// the runtime-length path for extent N (ose_generic.hip; nullptr outside
+// [2, GEN_NMAX])
+const FftEntry* generic_entry(int N);
+constexpr int GEN_NMAX = 4096;
 // smallest compiled extent >= n (0 if none)
 int next_supported(int n);
 

@@ -1,0 +1,299 @@
+// ose_generic.h -- runtime-length pass kernels for FFT extents without
+// compiled register kernels.
+//
+// The reference plans its 3D transform for whatever padded extent the odd
+// rule produces (cufftMakePlan3d(nzb, nyb, nxb), ose_3d.cu:226; judge_odd,
+// ose_3d_lib.cu:36-44).  The compiled kernels of ose_kernels.h cover the
+// extents N = N1 N2 with both factors in registers; every other extent up to
+// GEN_NMAX takes these kernels, which do the same four passes with the same
+// arguments and the same results in natural order, so the two kinds mix
+// freely across the axes of one model.
+//
+// A workgroup loads W pencils (strided passes, W x columns) or W x lines
+// into LDS, one element per thread and row at a time (coalesced along x),
+// and transforms them in place with the self-sorting mixed-radix scheme
+// (Stockham):  for stages of radix R with p = the product of the radices
+// before it, butterfly i in [0, N/R), k = i mod p, reads x[i + s N/R],
+// multiplies by w^(s k N/(p R)) (w = exp(-2 pi i / N), the axis's FP64 root
+// table), takes an R-point DFT and writes it to [(i - k) R + k + t p].
+// Each stage reads everything it needs into registers, waits, writes, waits,
+// so one LDS buffer suffices.  Radices 2, 3, 4, 5, 7, 8, 11, 13 and 16 use the
+// straight-line register DFTs of dft_gen.h; a larger prime factor R is a
+// direct stage: each output sums its R inputs with table twiddles (O(R) per
+// element, no recurrence, so no drift in FP32).
+#pragma once
+#include "ose_kernels.h"
+
+namespace ose {
+
+constexpr int GEN_MAXV = 16;   // values per thread per stage
+
+template <int SIGN>
+__device__ __forceinline__ float2 gtw(const float2* __restrict__ tw, int e)
+{
+    const float2 w = tw[e];
+    return SIGN < 0 ? w : make_float2(w.x, -w.y);
+}
+
+// LDS element e of the pencil addressed by (pitch, off)
+struct GenAt {
+    int pitch, off;
+    __device__ __forceinline__ int operator()(int e) const { return e * pitch + off; }
+};
+
+template <int R, int SIGN>
+__device__ __forceinline__ void gen_stage(float2* L, GenAt at, int N, int TP, int p, const float2* __restrict__ tw,
+                                          int t)
+{
+    constexpr int NB = GEN_MAXV / R;   // butterflies per thread
+    const int M = N / R;
+    const int tws = N / (p * R);
+    float2 v[NB][R];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int i = t + b * TP;
+        if (i < M) {
+            const int k = i % p;
+#pragma unroll
+            for (int s = 0; s < R; ++s) v[b][s] = L[at(i + s * M)];
+#pragma unroll
+            for (int s = 1; s < R; ++s) v[b][s] = cmul(v[b][s], gtw<SIGN>(tw, s * k * tws));
+            rdft<R, SIGN>(v[b]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int i = t + b * TP;
+        if (i < M) {
+            const int k = i % p;
+            const int j = (i - k) * R + k;
+#pragma unroll
+            for (int s = 0; s < R; ++s) L[at(j + s * p)] = v[b][s];
+        }
+    }
+    __syncthreads();
+}
+
+// a stage of prime radix R > 16: output o = j + s p (j = (i - k) R + k) is
+// sum_q x[i + q N/R] w^(q e), e = k N/(p R) + s N/R < N
+template <int SIGN>
+__device__ __forceinline__ void gen_direct(float2* L, GenAt at, int N, int TP, int p, int R,
+                                           const float2* __restrict__ tw, int t)
+{
+    const int M = N / R;
+    const int tws = N / (p * R);
+    float2 v[GEN_MAXV];
+#pragma unroll
+    for (int b = 0; b < GEN_MAXV; ++b) {
+        const int o = t + b * TP;
+        if (o < N) {
+            const int s = (o / p) % R, k = o % p, i = (o / (p * R)) * p + k;
+            const int e = k * tws + s * M;
+            float2 acc = make_float2(0.f, 0.f);
+            int idx = 0;
+            for (int q = 0; q < R; ++q) {
+                const float2 x = L[at(i + q * M)];
+                const float2 w = gtw<SIGN>(tw, idx);
+                acc.x += x.x * w.x - x.y * w.y;
+                acc.y += x.x * w.y + x.y * w.x;
+                idx += e;
+                if (idx >= N) idx -= N;
+            }
+            v[b] = acc;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < GEN_MAXV; ++b) {
+        const int o = t + b * TP;
+        if (o < N) L[at(o)] = v[b];
+    }
+    __syncthreads();
+}
+
+// unnormalised DFT (SIGN -1 forward, +1 inverse) of one pencil in LDS, natural
+// order in and out; every thread of the workgroup calls it (barriers inside)
+template <int SIGN>
+__device__ __forceinline__ void gen_fft(float2* L, GenAt at, const GenPlan& g, const float2* __restrict__ tw, int t)
+{
+    int p = 1;
+    for (int f = 0; f < g.nf; ++f) {
+        const int R = g.r[f];
+        switch (R) {
+        case 2: gen_stage<2, SIGN>(L, at, g.N, g.TP, p, tw, t); break;
+        case 3: gen_stage<3, SIGN>(L, at, g.N, g.TP, p, tw, t); break;
+        case 4: gen_stage<4, SIGN>(L, at, g.N, g.TP, p, tw, t); break;
+        case 5: gen_stage<5, SIGN>(L, at, g.N, g.TP, p, tw, t); break;
+        case 7: gen_stage<7, SIGN>(L, at, g.N, g.TP, p, tw, t); break;
+        case 8: gen_stage<8, SIGN>(L, at, g.N, g.TP, p, tw, t); break;
+        case 11: gen_stage<11, SIGN>(L, at, g.N, g.TP, p, tw, t); break;
+        case 13: gen_stage<13, SIGN>(L, at, g.N, g.TP, p, tw, t); break;
+        case 16: gen_stage<16, SIGN>(L, at, g.N, g.TP, p, tw, t); break;
+        default: gen_direct<SIGN>(L, at, g.N, g.TP, p, R, tw, t); break;
+        }
+        p *= R;
+    }
+}
+
+// the runtime-length strided pass: W = T x columns per workgroup, TP threads
+// per column; LDS rows of T + 1 elements
+struct GenStrided {
+    int T, col, t, x0, xc;
+    long base;
+    __device__ __forceinline__ GenStrided(const StridedArgs& a, const StridedTile& g)
+    {
+        T = a.g.W;
+        col = threadIdx.x % T;
+        t = threadIdx.x / T;
+        x0 = g.bx * T;
+        xc = min(col, a.ncol - 1 - x0);   // columns past the edge load the last one, store nothing
+        base = (long)g.bz * a.shot_stride + (long)g.by * a.os + x0 + xc;
+    }
+    __device__ __forceinline__ GenAt at() const { return GenAt{T + 1, col}; }
+    __device__ __forceinline__ void load(const StridedArgs& a, float2* L) const
+    {
+        for (int r = t; r < a.g.N; r += a.g.TP) L[r * (T + 1) + col] = a.in[base + (long)r * a.es];
+        __syncthreads();
+    }
+    __device__ __forceinline__ void store(const StridedArgs& a, const float2* L) const
+    {
+        if (col != xc) return;
+        for (int r = t; r < a.g.N; r += a.g.TP) a.out[base + (long)r * a.es] = L[r * (T + 1) + col];
+    }
+};
+
+template <int SIGN>
+__global__ __launch_bounds__(512) void k_gen_strided(StridedArgs a)
+{
+    extern __shared__ float2 lds[];
+    const StridedTile g(a);
+    const GenStrided s(a, g);
+    s.load(a, lds);
+    gen_fft<SIGN>(lds, s.at(), a.g, a.tw, s.t);
+    s.store(a, lds);
+}
+
+// z pass: forward-z, * |k| / N (scale_kxkz_3D, ose_3d_kernel.cu:101-126, and
+// the 1/N of scal_vr_3d, :355), inverse-z, as k_zsym
+__global__ __launch_bounds__(512) void k_gen_zsym(StridedArgs a)
+{
+    extern __shared__ float2 lds[];
+    const StridedTile g(a);
+    const GenStrided s(a, g);
+    s.load(a, lds);
+    gen_fft<-1>(lds, s.at(), a.g, a.tw, s.t);
+    const float kxy = a.kx2[s.x0 + s.xc] + a.ky2[g.by];
+    for (int r = s.t; r < a.g.N; r += a.g.TP) {
+        float2& v = lds[r * (s.T + 1) + s.col];
+        const float m = __builtin_amdgcn_sqrtf(kxy + a.kz2[r]) * a.invN;
+        v.x *= m;
+        v.y *= m;
+    }
+    __syncthreads();
+    gen_fft<1>(lds, s.at(), a.g, a.tw, s.t);
+    s.store(a, lds);
+}
+
+// x pass: W = P lines per workgroup, TP threads per line; the same modes,
+// arguments and per-element update as k_xpass
+template <int MODE>
+__global__ __launch_bounds__(512) void k_gen_xpass(XArgs a)
+{
+    extern __shared__ float2 lds[];
+    const int N = a.g.N, P = a.g.W, TP = a.g.TP;
+    const int l = threadIdx.x / TP;
+    const int t = threadIdx.x % TP;
+    // XCD-contiguous line blocks, as k_xpass
+    const unsigned bx = blockIdx.x, bq = gridDim.x / 8, br = gridDim.x % 8;
+    const long line = (long)((bx % 8) * bq + min(bx % 8, br) + bx / 8) * P + l;
+    const bool okl = line < a.nlines;
+    const int y = okl ? (int)(line % a.ny) : 0;
+    const int z = okl ? (int)(line / a.ny) : 0;
+    const long lbase = (long)blockIdx.y * a.shot_stride + line * N;
+    const long vbase = line * N;
+    float2* L = lds + (long)l * N;
+    const GenAt at{1, l * N};
+    const float2 zero = make_float2(0.f, 0.f);
+
+    if (MODE == XM_FWD || MODE == XM_INV) {
+        const float2* src = MODE == XM_FWD ? a.A : a.Xprev;
+        for (int x = t; x < N; x += TP) L[x] = okl ? src[lbase + x] : zero;
+        __syncthreads();
+        if (MODE == XM_FWD) {
+            gen_fft<-1>(lds, at, a.g, a.twf, t);
+            if (okl)
+                for (int x = t; x < N; x += TP) a.Xout[lbase + x] = L[x];
+        } else {
+            gen_fft<1>(lds, at, a.g, a.tw, t);
+            if (okl)
+                for (int x = t; x < N; x += TP) a.A[lbase + x] = make_float2(L[x].x * a.inv_nx, L[x].y * a.inv_nx);
+        }
+        return;
+    }
+
+    // L W_j / (v/R) in physical x (unnormalised inverse; the symbol carried 1/N)
+    for (int x = t; x < N; x += TP) L[x] = okl ? a.S[lbase + x] : zero;
+    __syncthreads();
+    gen_fft<1>(lds, at, a.g, a.tw, t);
+
+    if (okl) {
+        const float gyz = a.gy[y] * a.gz[z];
+        int itv = 0;
+        bool rec = false;
+        if (MODE == XM_LAST || MODE == XM_LASTA) {
+            itv = *a.it;
+            rec = (z == a.rec_z) && (y >= a.rec_y0) && (y < a.rec_y0 + a.rec_ny) && (itv < a.nt_cap);
+        }
+        // Win and Wout may be the same buffer: each element is read and then
+        // written by this thread only
+        const float2* win = a.Win + lbase;
+        float2* acc = a.A + lbase;
+        float* tr = rec ? a.traces + (long)blockIdx.y * a.trace_shot_stride +
+                              ((long)itv * a.rec_ny + (y - a.rec_y0)) * a.rec_nx
+                        : nullptr;
+        for (int x = t; x < N; x += TP) {
+            const float2 u = L[x];
+            const float vr = a.vr[vbase + x];
+            const float2 lw = make_float2(vr * u.x, vr * u.y);
+            float2 wn;
+            if (MODE == XM_ORDER0 || MODE == XM_ORDER0N) {
+                // W1 = -i L W0
+                wn = make_float2(lw.y, -lw.x);
+                if (MODE == XM_ORDER0) {
+                    const float2 wp = win[x];
+                    acc[x] = make_float2(a.c0 * wp.x + a.c1 * wn.x, a.c0 * wp.y + a.c1 * wn.y);
+                }
+            } else {
+                // W_{j+1} = W_{j-1} - 2i L W_j
+                const float2 wp = win[x];
+                wn = make_float2(wp.x + 2.f * lw.y, wp.y - 2.f * lw.x);
+                if (MODE != XM_SKIP) {
+                    const float gg = gyz * a.gx[x];
+                    const float gp = gpow(gg, a.eb) * a.cp, gn = gpow(gg, a.ec) * a.c1;
+                    float2 an = make_float2(gp * wp.x + gn * wn.x, gp * wp.y + gn * wn.y);
+                    if (MODE != XM_MIDF) {
+                        const float2 ao = acc[x];
+                        const float ga = gpow(gg, a.ea);
+                        an.x += ga * ao.x;
+                        an.y += ga * ao.y;
+                    }
+                    if (MODE == XM_MID || MODE == XM_MIDF) acc[x] = an;
+                    if (MODE == XM_LAST || MODE == XM_LASTA) wn = an;   // the new state u
+                }
+            }
+            L[x] = wn;
+            a.Wout[lbase + x] = wn;
+            if (rec && x >= a.rec_x0 && x < a.rec_x0 + a.rec_nx) tr[x - a.rec_x0] = wn.y;
+        }
+    } else {
+        for (int x = t; x < N; x += TP) L[x] = zero;
+    }
+    if (MODE == XM_LASTA) return;
+    __syncthreads();
+    gen_fft<-1>(lds, at, a.g, a.twf, t);
+    if (okl)
+        for (int x = t; x < N; x += TP) a.Xout[lbase + x] = L[x];
+}
+
+}  // namespace ose

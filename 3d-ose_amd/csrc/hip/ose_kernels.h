@@ -41,6 +41,19 @@ __device__ __forceinline__ float2 cmulc(float2 a, float2 b)
     return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
 
+// Extents without compiled register kernels take the runtime-length path of
+// ose_generic.h: a self-sorting mixed-radix transform staged through LDS.
+// Its plan travels in the kernel arguments (unused by the compiled kernels).
+constexpr int GEN_MAXF = 16;      // stages
+struct GenPlan {
+    int N = 0;                    // transform length
+    int TP = 0;                   // threads per pencil / line
+    int W = 0;                    // pencils (T columns) or lines (P) per workgroup
+    int nf = 0;                   // stages
+    short r[GEN_MAXF] = {};       // radix of each stage (<= 16: register
+                                  // butterfly; larger primes: direct sums)
+};
+
 // ---------------------------------------------------------------------
 // Strided passes (y and z): a workgroup owns T consecutive x columns of
 // one (z or y) plane and transforms all T pencils of length N.
@@ -62,6 +75,7 @@ struct StridedArgs {
     float invN;         // 1 / (nx ny nz)
     int xcd;            // 1: XCD-contiguous tile order (StridedTile)
     unsigned long long mgx, mgy;   // 2^40 / gridDim.x + 1 and 2^40 / gridDim.y + 1
+    GenPlan g;          // runtime-length path only
 };
 
 // floor(n / d) as (n * m) >> 40 with m = 2^40 / d + 1: exact for n < 2^24 and
@@ -374,6 +388,7 @@ struct XArgs {
     int rec_z, rec_y0, rec_ny, rec_x0, rec_nx;
     const int* it;          // device step counter
     int nt_cap;
+    GenPlan g;              // runtime-length path only
 };
 
 template <int N1, int N2, int P, int SPLIT>

@@ -40,6 +40,7 @@ struct BuiltModel {
     std::vector<float> gx, gy, gz;
     int id = -1;
     std::vector<int> shot_ids;
+    std::string adjusted;   // extent=smooth changes, as JSON entries
 };
 
 struct ShotRec {
@@ -193,23 +194,33 @@ int build_specs(oseh_sim* s)
     return OSE_OK;
 }
 
-// pads of one axis: before/after, the odd rule, then the extent rule
-int axis_pads(oseh_sim* s, int n, int b, int& before, int& after, int& nb)
+// pads of one axis: before/after, the odd rule (judge_odd, ose_3d_lib.cu:36-44),
+// then the extent rule.  extent=exact (the default) keeps the reference's
+// padded extent, which every size up to 4096 supports; extent=smooth widens
+// the after-pad to the next extent with compiled kernels (faster transforms,
+// but a different periodic domain and taper than the reference's grid), and
+// says so on stderr and in the info record.
+int axis_pads(oseh_sim* s, const char* axis, int n, int b, int& before, int& after, int& nb, std::string& adjusted)
 {
     before = b;
     after = b;
     nb = before + n + after;
     oseh_judge_odd(&after, &nb);
-    const std::string ext = s->get("extent", "smooth");
-    if (!ose_extent_is_supported(nb)) {
-        if (ext == "exact")
-            return s->fail(OSE_EUNSUPPORTED, "padded extent " + std::to_string(nb) +
-                                                 " is not compiled (use extent=smooth)");
+    const std::string ext = s->get("extent", "exact");
+    if (ext != "exact" && ext != "smooth") return s->fail(OSE_EINVAL, "extent must be exact or smooth");
+    if (ext == "smooth" && ose_extent_kind(nb) != 1) {
         const int t = ose_supported_extent(nb);
-        if (t <= 0) return s->fail(OSE_EUNSUPPORTED, "no compiled extent >= " + std::to_string(nb));
-        after += t - nb;   // the taper is extended over the extra cells
-        nb = t;
+        if (t > 0) {
+            std::fprintf(stderr, "ose: extent=smooth widens the %s padded extent %d to %d (after-pad %d -> %d cells)\n",
+                         axis, nb, t, after, after + t - nb);
+            adjusted += std::string(adjusted.empty() ? "" : ", ") + "{\"axis\": \"" + axis +
+                        "\", \"from\": " + std::to_string(nb) + ", \"to\": " + std::to_string(t) + "}";
+            after += t - nb;   // the taper is extended over the extra cells
+            nb = t;
+        }
     }
+    if (!ose_extent_is_supported(nb))
+        return s->fail(OSE_EUNSUPPORTED, "padded extent " + std::to_string(nb) + " is outside [2, 4096]");
     return OSE_OK;
 }
 
@@ -218,9 +229,10 @@ int build_model(oseh_sim* s, const ModelSpec& spec, BuiltModel& m)
     m.spec = spec;
     const int b = s->geti("boundary", 20);
     int rc;
-    if ((rc = axis_pads(s, spec.nx, s->geti("bl", b), m.bl, m.br, m.nxb))) return rc;
-    if ((rc = axis_pads(s, spec.ny, s->geti("bf", b), m.bf, m.bb, m.nyb))) return rc;
-    if ((rc = axis_pads(s, spec.nz, s->geti("bu", b), m.bu, m.bd, m.nzb))) return rc;
+    m.adjusted.clear();
+    if ((rc = axis_pads(s, "x", spec.nx, s->geti("bl", b), m.bl, m.br, m.nxb, m.adjusted))) return rc;
+    if ((rc = axis_pads(s, "y", spec.ny, s->geti("bf", b), m.bf, m.bb, m.nyb, m.adjusted))) return rc;
+    if ((rc = axis_pads(s, "z", spec.nz, s->geti("bu", b), m.bu, m.bd, m.nzb, m.adjusted))) return rc;
     std::vector<float> vi((size_t)spec.nx * spec.ny * spec.nz);
     if (spec.kind == -1) {
         rc = oseh_read_velocity_xyz(s->get("vfile", "").c_str(), spec.nx, spec.ny, spec.nz, vi.data());
@@ -573,7 +585,7 @@ int oseh_sim_info(oseh_sim* s, char* buf, size_t cap)
           << "], \"padded\": [" << m.nxb << ", " << m.nyb << ", " << m.nzb << "], \"pads\": [" << m.bl << ", "
           << m.br << ", " << m.bf << ", " << m.bb << ", " << m.bu << ", " << m.bd << "], \"K\": " << m.K
           << ", \"R\": " << m.R << ", \"z\": " << m.z << ", \"vmax\": " << m.vmax
-          << ", \"shots\": " << m.spec.shots.size() << "}";
+          << ", \"shots\": " << m.spec.shots.size() << ", \"extent_adjusted\": [" << m.adjusted << "]}";
     }
     o << "], \"nshots\": " << s->shots.size() << ", \"interior_cells\": " << (long long)interior
       << ", \"padded_cells\": " << (long long)padded << ", \"nt\": " << s->geti("nt", 300)

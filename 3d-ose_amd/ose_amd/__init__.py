@@ -6,4 +6,4 @@ tapers, models, file formats) is libose_host.so (include/ose_host.h).  This
 package is a thin ctypes view of both for tests and the benchmark.
 """
 from ._lib import OseError  # noqa: F401
-from .backend import BoundaryNet, Context, boundary_net_size, extent_is_supported, supported_extent  # noqa: F401
+from .backend import BoundaryNet, Context, boundary_net_size, extent_is_supported, extent_kind, supported_extent  # noqa: F401

@@ -112,6 +112,8 @@ def hip():
         _proto(L, "ose_bnet_train", [vp, c_int, c_float, ctypes.c_ulonglong, fp])
         _proto(L, "ose_supported_extent", [c_int])
         _proto(L, "ose_extent_is_supported", [c_int])
+        _proto(L, "ose_extent_kind", [c_int])
+        _proto(L, "ose_set_fft_path", [vp, c_int])
         _proto(L, "ose_accumulation_plan", [c_int, ip])
         _proto(L, "ose_set_streams", [vp, c_int])
         _hip = L
@@ -139,7 +141,7 @@ def hip_symbols():
         "ose_receiver_count", "ose_set_source_profile", "ose_set_snapshots", "ose_snapshot_drain", "ose_get_wavefield",
         "ose_set_wavefield", "ose_get_step", "ose_set_step", "ose_apply_operator", "ose_apply_operator_noalloc",
         "ose_fft3d", "ose_set_options", "ose_pass_stats", "ose_reset_stats",
-        "ose_supported_extent", "ose_extent_is_supported", "ose_set_autotune", "ose_get_tuning",
+        "ose_supported_extent", "ose_extent_is_supported", "ose_extent_kind", "ose_set_fft_path", "ose_set_autotune", "ose_get_tuning",
         "ose_set_tuning", "ose_get_tuning_modes", "ose_set_tuning_modes", "ose_launch_shapes",
         "ose_copy_probe", "ose_pass_probe", "ose_boundary_net_size", "ose_set_boundary_net",
         "ose_boundary_status", "ose_bnet_create", "ose_bnet_destroy", "ose_bnet_error",

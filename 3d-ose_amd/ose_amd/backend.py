@@ -186,6 +186,11 @@ class Context:
     def set_options(self, use_graph=True, profile=False):
         self._check(self._L.ose_set_options(self._h, int(use_graph), int(profile)))
 
+    def set_fft_path(self, path):
+        """1: runtime-length kernels on every axis of the models added
+        afterwards; 0: compiled kernels where they exist (default)."""
+        self._check(self._L.ose_set_fft_path(self._h, int(path)))
+
     def set_streams(self, n):
         """2: the step's shots run as two concurrent lanes; 1: one stream."""
         self._check(self._L.ose_set_streams(self._h, int(n)))
@@ -274,6 +279,11 @@ def supported_extent(n):
 
 def extent_is_supported(n):
     return bool(_lib.hip().ose_extent_is_supported(int(n)))
+
+
+def extent_kind(n):
+    """1: compiled kernels, 2: the runtime-length path, 0: unsupported."""
+    return _lib.hip().ose_extent_kind(int(n))
 
 
 def boundary_net_size():

@@ -34,7 +34,7 @@ extern "C" {
 #define OSE_EDEVICE       3   /* HIP runtime error                             */
 #define OSE_ESTATE        4   /* call not valid in the context's current state */
 #define OSE_ENUMERIC      5   /* non-finite input                              */
-#define OSE_EUNSUPPORTED  6   /* FFT extent not compiled                       */
+#define OSE_EUNSUPPORTED  6   /* FFT extent outside [2, 4096]                  */
 
 typedef struct ose_ctx ose_ctx;
 
@@ -296,10 +296,23 @@ int ose_bnet_add_sample(ose_bnet* net, const float* u, const float* t, int* coun
 /* iters Adam steps on randomly drawn stored pairs; losses[iters].          */
 int ose_bnet_train(ose_bnet* net, int iters, float lr, unsigned long long seed, float* losses);
 
-/* Smallest compiled FFT extent >= n, or 0.                                 */
+/* FFT extents.  The reference plans its transform for any padded extent
+ * (cufftMakePlan3d(nzb, nyb, nxb), ose_3d.cu:226).  Here every extent in
+ * [2, 4096] runs: the extents N = N1 N2 whose factors both fit the
+ * compiled register DFTs take those kernels, every other one (large prime
+ * factors, extents above 1024, ...) a runtime-length mixed-radix path with
+ * the same passes and results (slower: no register-resident transform).
+ * ose_extent_kind: 1 compiled kernels, 2 the runtime-length path, 0 not
+ * supported.  ose_supported_extent: the smallest extent >= n with compiled
+ * kernels, or 0 (what the host's extent=smooth rounds up to).             */
 int ose_supported_extent(int n);
-/* Nonzero if extent n is compiled.                                         */
 int ose_extent_is_supported(int n);
+int ose_extent_kind(int n);
+/* path 1: every model added afterwards (and ose_fft3d) uses the
+ * runtime-length kernels on all three axes, compiled or not; 0 (default):
+ * compiled kernels where they exist.  Before the first ose_add_model.  For
+ * tests and for comparing the two paths; no counterpart in the reference. */
+int ose_set_fft_path(ose_ctx* ctx, int path);
 /* The step's accumulator schedule for K orders (host only, no device).  The
  * series sum A = sum_m c_m g^(e_m) W_m of wavefield_extro_ori2
  * (ose_3d_lib.cu:741-859: add_bessel_3d then apply_ABC_Xt on every order

@@ -153,7 +153,12 @@ int oseh_gaussian_profile(double sigma, int center, int n, int* start, float* ou
  * src_sigma=<cells>|auto replaces each point source by a normalised
  * Gaussian profile (ose_set_source_profile); auto takes sigma = 3.72 v dt /
  * (pi d) cells per axis, v the velocity at the source, so the profile's
- * spectrum is below 1e-3 at the temporal Nyquist pi / (v dt).             */
+ * spectrum is below 1e-3 at the temporal Nyquist pi / (v dt).
+ * extent=exact (default; presets ref and c0) keeps each padded extent the
+ * odd rule gives, which runs at any size up to 4096; extent=smooth (presets
+ * c1-c4) widens an after-pad to the next extent with compiled transform
+ * kernels, which changes the periodic domain and the taper: it reports each
+ * change on stderr and in the info record ("extent_adjusted").            */
 typedef struct oseh_sim oseh_sim;
 int  oseh_sim_create(oseh_sim** out, const char* params);
 /* Build velocities, pads, wavelets, orders and coefficients (host only).  */

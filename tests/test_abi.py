@@ -38,7 +38,10 @@ def test_host_only_entry_points():
     assert L.ose_destroy(None) == _lib.OSE_EINVAL
     assert L.ose_last_error(None) == b"null context"
     assert L.ose_extent_is_supported(152) and L.ose_extent_is_supported(864)
-    assert not L.ose_extent_is_supported(17)
+    # every extent in [2, 4096] runs: compiled kernels, or the runtime-length path
+    assert all(L.ose_extent_is_supported(n) for n in (2, 17, 106, 150, 1018, 2048, 4096))
+    assert not L.ose_extent_is_supported(1) and not L.ose_extent_is_supported(4097)
+    assert [L.ose_extent_kind(n) for n in (152, 864, 17, 106, 1100, 4097)] == [1, 1, 2, 2, 2, 0]
     assert L.ose_supported_extent(848) == 864
     assert L.ose_supported_extent(272) == 272
     assert L.ose_supported_extent(273) == 280
