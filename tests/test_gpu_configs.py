@@ -62,6 +62,38 @@ def test_c1_seismogram_matches_fp64_oracle_at_full_size():
     assert abs(ratio - 1.0) < 1e-4
 
 
+def test_reference_as_shipped_matches_fp64_oracle():
+    """The reference exactly as shipped (ose_3d.cu:75-125, 157-160,
+    242-279): its 101^3 two-layer velocity file (the preset's model equals
+    the file, checked when the fixture was made; SHA-256 recorded), 25-cell
+    pads -> 152^3, all 300 steps, K = M + 1, generate_att taper,
+    unnormalised Hilbert Q-source, receivers on the whole plane z = 0.
+    FP32 GPU against FP64 oracle (tests/golden/gen_ref_oracle.py): rel-L2
+    over every second receiver in x and y, the L2 norm of every one of the
+    10201 traces, and the norm of the whole array."""
+    from ose_amd.host import Simulation
+    g = np.load(os.path.join(GOLD, "ref_oracle.npz"))
+    nt = int(g["nt"])
+    print(f"reference as shipped, {nt} steps", flush=True)
+    sim = Simulation(f"preset=ref nt={nt}").setup()
+    try:
+        m = sim.model(0)
+        assert m["K"] == int(g["K"]) and m["dims"][9:12] == [152, 152, 152]
+        sim.run(nt)
+        tr = sim.traces(0, 0, nt)
+    finally:
+        sim.close()
+    err = rel(tr[g["idx"]], g["traces"])
+    norms = np.linalg.norm(tr.astype(np.float64), axis=1)
+    dev = float(np.max(np.abs(norms - g["trace_norms"])) / np.max(g["trace_norms"]))
+    ratio = np.linalg.norm(tr.astype(np.float64)) / float(g["norm_full"])
+    print(f"ref rel-L2 (every 2nd receiver) {err:.3g}; max trace-norm deviation {dev:.3g} of the largest; "
+          f"full-array norm ratio {ratio:.7f}", flush=True)
+    assert err < 1e-4
+    assert dev < 1e-4
+    assert abs(ratio - 1.0) < 1e-4
+
+
 def test_trained_thin_pad_beats_plain_taper():
     """Held-out seeded model (64^3 interior): the committed 8-cell learned
     pad against the 8-cell reference taper, both measured against a 64-cell

@@ -98,3 +98,39 @@ def test_runtime_length_path_equals_compiled_path(K):
         assert np.abs(t0).max() > 0
         assert rel(t1, t0) < 1e-5
         assert rel(r1 + 1j * i1, r0 + 1j * i0) < 1e-5
+
+
+@pytest.mark.parametrize("name,dims,padded", [("v3d_nx100_ny101_nz102", (100, 101, 102), (150, 152, 152)),
+                                              ("3d_number-100-100-100-3", (100, 100, 100), (150, 150, 150))])
+def test_reference_input_files_on_their_own_grid(name, dims, padded, tmp_path):
+    """The reference's two other velocity files at its 25-cell padding,
+    extent=exact, so x (and for the 100^3 file every axis) is 150 cells on
+    the runtime-length path: 60 steps against the FP64 oracle
+    (tests/golden/gen_refinput_oracle.py; the files are stored there
+    byte for byte)."""
+    import os
+    from ose_amd.host import Simulation
+    gold = os.path.join(os.path.dirname(__file__), "golden")
+    vin = np.load(os.path.join(gold, "ref_inputs.npz"))
+    g = np.load(os.path.join(gold, "refinput_oracle.npz"))
+    nt = int(g["nt"])
+    vfile = tmp_path / (name + ".bin")
+    vin[name].astype("<f4").tofile(vfile)
+    nx, ny, nz = dims
+    print(f"{name} at boundary 25, {nt} steps", flush=True)
+    sim = Simulation(f"preset=ref nx={nx} ny={ny} nz={nz} model=file vfile={vfile} nt={nt} delay=0 "
+                     f"sx=50 sy=50 sz=0 extent=exact").setup()
+    try:
+        m = sim.model(0)
+        assert tuple(m["dims"][9:12]) == padded == tuple(g[name + "_padded"])
+        assert m["K"] == int(g[name + "_K"])
+        sim.run(nt)
+        tr = sim.traces(0, 0, nt)
+    finally:
+        sim.close()
+    err = rel(tr[g[name + "_idx"]], g[name + "_traces"])
+    norms = np.linalg.norm(tr.astype(np.float64), axis=1)
+    dev = float(np.max(np.abs(norms - g[name + "_trace_norms"])) / np.max(g[name + "_trace_norms"]))
+    ratio = np.linalg.norm(tr.astype(np.float64)) / float(g[name + "_norm_full"])
+    print(f"  rel-L2 {err:.3g}, trace-norm deviation {dev:.3g}, norm ratio {ratio:.7f}", flush=True)
+    assert err < 1e-4 and dev < 1e-4 and abs(ratio - 1.0) < 1e-4

@@ -177,6 +177,38 @@ def test_reference_preset_parameters(H):
     sim.close()
 
 
+def test_extent_rule_exact_and_smooth(H):
+    """extent=exact (the default) keeps the odd rule's padded extent, any
+    size up to 4096, e.g. the 150 of the reference's 100-cell file at 25-cell
+    pads; extent=smooth widens it to the next compiled extent and records it."""
+    sim = H.Simulation("nx=100 ny=101 nz=102 boundary=25 model=const nt=4").build()
+    assert sim.model_dims(0)[9:12] == [150, 152, 152]
+    assert sim.info()["models"][0]["extent_adjusted"] == []
+    sim.close()
+    sim = H.Simulation("nx=100 ny=101 nz=102 boundary=25 model=const nt=4 extent=smooth").build()
+    d = sim.model_dims(0)
+    import ose_amd
+    assert d[9] == ose_amd.supported_extent(150) > 150 and d[4] == 25 + d[9] - 150
+    assert sim.info()["models"][0]["extent_adjusted"] == [{"axis": "x", "from": 150, "to": d[9]}]
+    sim.close()
+    with pytest.raises(Exception):
+        H.Simulation("nx=4090 ny=8 nz=8 boundary=4 model=const nt=4").build()
+
+
+def test_reference_input_fixture_is_the_file():
+    """tests/golden/ref_inputs.npz holds the reference's velocity files byte
+    for byte (their SHA-256 recorded when the fixture was made), and matches
+    the files themselves where the reference is present."""
+    import hashlib
+    g = np.load(os.path.join(GOLD, "ref_inputs.npz"))
+    for name in ("v3d_nx100_ny101_nz102", "3d_number-100-100-100-3"):
+        raw = g[name].astype("<f4").tobytes()
+        assert hashlib.sha256(raw).hexdigest() == str(g[name + "_sha256"])
+        path = os.path.join(REF_INPUT, name + ".bin")
+        if os.path.exists(path):
+            assert open(path, "rb").read() == raw
+
+
 def test_c2_batch_layout(H):
     sim = H.Simulation("preset=c2").build()
     assert sim.nmodels() == 5
