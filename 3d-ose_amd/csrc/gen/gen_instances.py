@@ -6,10 +6,12 @@ split and fills an ose::FftEntry with their addresses, block sizes and LDS
 sizes.  It also writes dft_gen.h (the register DFTs for every N1, N2) and
 ose_sizes.inc (the registry list).
 
-Supported extents are the even 2,3,5,7-smooth sizes the configs need (plus
-152 = 8*19, the reference's own 101+25+26 grid).  Extents outside the list
-are rejected by ose_add_model with OSE_EUNSUPPORTED; the host's extent
-chooser rounds a padded size up to the next supported one.
+SIZES are the extents the configurations use (plus 152 = 8*19, the
+reference's own 101+25+26 grid), compiled in every launch shape the
+autotuner chooses from; LEAN are the other even extents up to 1024 with
+small prime factors, one shape each.  Every other extent up to 4096 runs
+on the runtime-length path (ose_generic.h); the host's extent=smooth rounds
+a padded size up to the next compiled one.
 
 Run: python gen_instances.py <outdir>
 """
@@ -34,6 +36,33 @@ SIZES = {
 
 # radices of the runtime-length path's register butterflies (ose_generic.h)
 GENERIC_RADICES = (2, 3, 4, 5, 7, 8, 11, 13, 16)
+
+def _prime_factors(n):
+    out, p = [], 2
+    while n > 1:
+        while n % p == 0:
+            out.append(p)
+            n //= p
+        p += 1
+    return out
+
+
+def _split(n):
+    """The most balanced N1 * N2 = n with 2 <= N1 <= N2 <= 32, or None."""
+    best = None
+    for a in range(2, 33):
+        b, r = divmod(n, a)
+        if r == 0 and a <= b <= 32 and (best is None or b - a < best[1] - best[0]):
+            best = (a, b)
+    return best
+
+
+# Lean instances: every other even extent up to 1024 whose prime factors are
+# at most 13 and that splits into two register DFTs, compiled with a single
+# launch shape per pass (no autotuner choice) to keep the library small.
+# Extents outside both lists take the runtime-length path (ose_generic.h).
+LEAN = {n: _split(n) for n in range(16, 1025, 2)
+        if n not in SIZES and _split(n) and max(_prime_factors(n)) <= 13}
 
 STRIDED_TS = (8, 16, 32)
 LDS_MAX = 160 * 1024
@@ -128,11 +157,23 @@ TAIL = """}}
 """
 
 
-def instance_source(n, n1, n2):
+def lean_variants(n1, n2):
+    """One strided and one x-pass shape: T = 16 (split exchange above 40 KiB
+    of LDS) and P = 128 / NT lines (split above 2 KiB per line)."""
+    nt = max(n1, n2)
+    full = 8 * n1 * n2 * 17
+    strided = [(16, 1 if full > 40 * 1024 else 0)]
+    p = max(1, 128 // nt)
+    xs = [(p, 1 if 8 * n2 * (n1 | 1) > 2048 else 0, 0)]
+    return strided, xs
+
+
+def instance_source(n, n1, n2, lean=False):
     src = HEAD.format(N=n, N1=n1, N2=n2)
-    for t, split in strided_variants(n1, n2):
+    strided, xs = lean_variants(n1, n2) if lean else (strided_variants(n1, n2), x_variants(n1, n2))
+    for t, split in strided:
         src += STRIDED.format(T=t, SPLIT=split)
-    for p, split, twb in x_variants(n1, n2):
+    for p, split, twb in xs:
         src += XPASS.format(P=p, SPLIT=split, TWB=twb)
     return src + TAIL.format()
 
@@ -140,11 +181,12 @@ def instance_source(n, n1, n2):
 def main():
     out = sys.argv[1]
     only = [int(a) for a in sys.argv[2:]]
-    sizes = {n: SIZES[n] for n in (only or SIZES)}
+    allsizes = {**SIZES, **LEAN}
+    sizes = {n: allsizes[n] for n in (only or allsizes)}
     os.makedirs(out, exist_ok=True)
     # register DFTs of the compiled splits and of the runtime-length path's
     # radices (ose_generic.h)
-    factors = sorted({f for (a, b) in SIZES.values() for f in (a, b)} | set(GENERIC_RADICES))
+    factors = sorted({f for (a, b) in allsizes.values() for f in (a, b)} | set(GENERIC_RADICES))
     subprocess.check_call([sys.executable, os.path.join(HERE, "gen_dft.py"),
                            os.path.join(out, "dft_gen.h")] + [str(f) for f in factors])
     with open(os.path.join(out, "ose_sizes.inc"), "w") as f:
@@ -153,7 +195,7 @@ def main():
             f.write(f"OSE_SIZE({n})\n")
     for n in sorted(sizes):
         n1, n2 = sizes[n]
-        src = instance_source(n, n1, n2)
+        src = instance_source(n, n1, n2, lean=n in LEAN)
         path = os.path.join(out, f"inst_{n}.hip")
         old = open(path).read() if os.path.exists(path) else None
         if old != src:

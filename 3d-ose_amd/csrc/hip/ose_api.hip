@@ -186,13 +186,23 @@ const FftEntry* find_entry(int N)
     return e ? e : generic_entry(N);
 }
 
-// smallest extent >= n with compiled kernels (the extent=smooth rounding)
+// compiled in several launch shapes (the autotuner chooses), not a single one
+static bool tuned(const FftEntry& e) { return e.ns > 1 || e.nx > 1; }
+
+// smallest extent >= n compiled in several launch shapes (the extent=smooth rounding)
 int next_supported(int n)
 {
     int best = 0;
     for (auto& e : registry().entries)
-        if (e.N >= n && (best == 0 || e.N < best)) best = e.N;
+        if (tuned(e) && e.N >= n && (best == 0 || e.N < best)) best = e.N;
     return best;
+}
+
+int extent_kind(int n)
+{
+    const FftEntry* e = find_compiled(n);
+    if (e) return tuned(*e) ? 1 : 3;
+    return generic_entry(n) ? 2 : 0;
 }
 
 }  // namespace ose
@@ -1623,7 +1633,7 @@ int ose_accumulation_plan(int K, int* plan)
     return OSE_OK;
 }
 int ose_extent_is_supported(int n) { return find_entry(n) != nullptr; }
-int ose_extent_kind(int n) { return find_compiled(n) ? 1 : generic_entry(n) ? 2 : 0; }
+int ose_extent_kind(int n) { return extent_kind(n); }
 
 int ose_set_fft_path(ose_ctx* c, int path)
 {

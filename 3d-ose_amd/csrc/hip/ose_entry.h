@@ -46,7 +46,10 @@ const FftEntry* find_compiled(int N);
 // [2, GEN_NMAX])
 const FftEntry* generic_entry(int N);
 constexpr int GEN_NMAX = 4096;
-// smallest compiled extent >= n (0 if none)
+// smallest extent >= n compiled in several launch shapes (0 if none)
 int next_supported(int n);
+// 1: compiled in several launch shapes, 3: compiled in one, 2: the
+// runtime-length path, 0: unsupported
+int extent_kind(int n);
 
 }  // namespace ose

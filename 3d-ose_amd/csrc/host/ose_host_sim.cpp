@@ -197,9 +197,9 @@ int build_specs(oseh_sim* s)
 // pads of one axis: before/after, the odd rule (judge_odd, ose_3d_lib.cu:36-44),
 // then the extent rule.  extent=exact (the default) keeps the reference's
 // padded extent, which every size up to 4096 supports; extent=smooth widens
-// the after-pad to the next extent with compiled kernels (faster transforms,
-// but a different periodic domain and taper than the reference's grid), and
-// says so on stderr and in the info record.
+// the after-pad to the next extent compiled in autotuned launch shapes
+// (faster transforms, but a different periodic domain and taper than the
+// reference's grid), and says so on stderr and in the info record.
 int axis_pads(oseh_sim* s, const char* axis, int n, int b, int& before, int& after, int& nb, std::string& adjusted)
 {
     before = b;
@@ -208,7 +208,7 @@ int axis_pads(oseh_sim* s, const char* axis, int n, int b, int& before, int& aft
     oseh_judge_odd(&after, &nb);
     const std::string ext = s->get("extent", "exact");
     if (ext != "exact" && ext != "smooth") return s->fail(OSE_EINVAL, "extent must be exact or smooth");
-    if (ext == "smooth" && ose_extent_kind(nb) != 1) {
+    if (ext == "smooth" && ose_extent_kind(nb) != 1) {   // not compiled in autotuned launch shapes
         const int t = ose_supported_extent(nb);
         if (t > 0) {
             std::fprintf(stderr, "ose: extent=smooth widens the %s padded extent %d to %d (after-pad %d -> %d cells)\n",

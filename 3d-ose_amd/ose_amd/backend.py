@@ -282,7 +282,8 @@ def extent_is_supported(n):
 
 
 def extent_kind(n):
-    """1: compiled kernels, 2: the runtime-length path, 0: unsupported."""
+    """1: compiled in several launch shapes, 3: compiled in one, 2: the
+    runtime-length path, 0: unsupported."""
     return _lib.hip().ose_extent_kind(int(n))
 
 

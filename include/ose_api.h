@@ -302,9 +302,10 @@ int ose_bnet_train(ose_bnet* net, int iters, float lr, unsigned long long seed, 
  * compiled register DFTs take those kernels, every other one (large prime
  * factors, extents above 1024, ...) a runtime-length mixed-radix path with
  * the same passes and results (slower: no register-resident transform).
- * ose_extent_kind: 1 compiled kernels, 2 the runtime-length path, 0 not
- * supported.  ose_supported_extent: the smallest extent >= n with compiled
- * kernels, or 0 (what the host's extent=smooth rounds up to).             */
+ * ose_extent_kind: 1 compiled in several launch shapes (the autotuner picks
+ * one), 3 compiled in a single launch shape, 2 the runtime-length path, 0
+ * not supported.  ose_supported_extent: the smallest extent >= n of kind 1,
+ * or 0 (what the host's extent=smooth rounds up to).                      */
 int ose_supported_extent(int n);
 int ose_extent_is_supported(int n);
 int ose_extent_kind(int n);

@@ -156,7 +156,7 @@ int oseh_gaussian_profile(double sigma, int center, int n, int* start, float* ou
  * spectrum is below 1e-3 at the temporal Nyquist pi / (v dt).
  * extent=exact (default; presets ref and c0) keeps each padded extent the
  * odd rule gives, which runs at any size up to 4096; extent=smooth (presets
- * c1-c4) widens an after-pad to the next extent with compiled transform
+ * c1-c4) widens an after-pad to the next extent with autotuned compiled
  * kernels, which changes the periodic domain and the taper: it reports each
  * change on stderr and in the info record ("extent_adjusted").            */
 typedef struct oseh_sim oseh_sim;

@@ -7,7 +7,10 @@ with 25-cell pads, or 106 = 2 * 53.  Checked here: 3D transforms against
 numpy at extents with large prime factors and above 1024; one OSE step
 against the FP64 oracle with such an extent on each axis; the
 runtime-length path forced onto compiled extents against the compiled
-kernels over a batched run with receivers (every x-pass mode)."""
+kernels over a batched run with receivers (every x-pass mode); every
+extent compiled with a single launch shape against numpy and, on a few
+grids, one step against the oracle; the reference's own input files on
+their own grids against the FP64 oracle."""
 import numpy as np
 import pytest
 
@@ -21,7 +24,7 @@ def _cfield(shape, seed):
     return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
 
 
-@pytest.mark.parametrize("shape", [(20, 30, 106), (26, 150, 40), (1100, 8, 6), (4, 6, 2042), (2048, 4, 2),
+@pytest.mark.parametrize("shape", [(20, 30, 106), (26, 158, 40), (1100, 8, 6), (4, 6, 2042), (2048, 4, 2),
                                    (3, 4096, 2), (13, 17, 19)])
 def test_fft3d_runtime_length_matches_numpy(shape):
     import ose_amd
@@ -45,11 +48,11 @@ def test_forced_runtime_length_fft_matches_numpy_on_compiled_extents():
     assert rel(f, np.fft.fftn(a.astype(np.complex128))) < 1e-5
 
 
-@pytest.mark.parametrize("interior", [(94, 36, 28), (36, 138, 28), (28, 24, 1088), (94, 138, 46)])
+@pytest.mark.parametrize("interior", [(94, 36, 28), (36, 94, 28), (28, 24, 1088), (94, 106, 46)])
 def test_step_with_runtime_length_extents_matches_oracle(interior):
     """One step from a smooth random state, reference taper, K = M + 1, on
-    padded grids 106 x 48 x 40, 48 x 150 x 40, 40 x 36 x 1100 and
-    106 x 150 x 58 (the runtime-length path on x, y, z and all three)."""
+    padded grids 106 x 48 x 40, 48 x 106 x 40, 40 x 36 x 1100 and
+    106 x 118 x 58 (the runtime-length path on x, y, z and all three)."""
     import ose_amd
     print(f"one step, padded {tuple(n + 12 for n in interior)}", flush=True)
     case = Case(interior, (6, 6, 6, 6, 6, 6), layered(*interior, seed=9))
@@ -134,3 +137,52 @@ def test_reference_input_files_on_their_own_grid(name, dims, padded, tmp_path):
     ratio = np.linalg.norm(tr.astype(np.float64)) / float(g[name + "_norm_full"])
     print(f"  rel-L2 {err:.3g}, trace-norm deviation {dev:.3g}, norm ratio {ratio:.7f}", flush=True)
     assert err < 1e-4 and dev < 1e-4 and abs(ratio - 1.0) < 1e-4
+
+
+def test_every_lean_extent_transforms_like_numpy():
+    """Every extent compiled with a single launch shape (gen_instances.LEAN:
+    even, up to 1024, prime factors <= 13) on each axis of a 3D transform."""
+    import os
+    import sys
+    import ose_amd
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "3d-ose_amd",
+                                    "csrc", "gen"))
+    import gen_instances as G
+    sizes = sorted(G.LEAN)
+    print(f"{len(sizes)} lean extents", flush=True)
+    worst = 0.0
+    with ose_amd.Context(0) as c:
+        for n in sizes:
+            assert ose_amd.extent_kind(n) == 3
+            for shape in ((2, 4, n), (2, n, 4), (n, 4, 2)):
+                a = _cfield(shape, n)
+                f = c.fft3d(a, -1)
+                b = c.fft3d(f, +1)
+                e = max(rel(f, np.fft.fftn(a.astype(np.complex128))), rel(b, a.astype(np.complex128) * a.size))
+                worst = max(worst, e)
+                assert e < 1e-5, (n, shape, e)
+    print(f"  worst rel-L2 {worst:.2e}", flush=True)
+
+
+@pytest.mark.parametrize("padded", [(150, 90, 130), (338, 22, 40), (48, 286, 18), (40, 36, 896)])
+def test_step_on_lean_extents_matches_oracle(padded):
+    """One step on grids whose extents are compiled with a single shape."""
+    import ose_amd
+    interior = tuple(n - 12 for n in padded)
+    print(f"one step, padded {padded}", flush=True)
+    case = Case(interior, (6, 6, 6, 6, 6, 6), layered(*interior, seed=13))
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal(case.shape) + 1j * rng.standard_normal(case.shape)
+    F = np.fft.fftn(u)
+    kz, ky, kx = np.meshgrid(*[np.fft.fftfreq(n) for n in case.shape], indexing="ij")
+    u = np.fft.ifftn(F * np.exp(-(kx ** 2 + ky ** 2 + kz ** 2) / 0.03))
+    with ose_amd.Context(0) as c:
+        m = case.add_to(c)
+        sid = c.add_shot(m, (case.bl, case.bf, case.bu), np.zeros(0), np.zeros(0))
+        c.set_wavefield(sid, u.real, u.imag)
+        c.run(1)
+        re, im = c.wavefield(sid)
+    P, Q = case.oracle().step(u.real, u.imag)
+    err = rel(re + 1j * im, P + 1j * Q)
+    print(f"  rel-L2 vs FP64 oracle {err:.2e}", flush=True)
+    assert err < 1e-5

@@ -180,16 +180,21 @@ def test_reference_preset_parameters(H):
 def test_extent_rule_exact_and_smooth(H):
     """extent=exact (the default) keeps the odd rule's padded extent, any
     size up to 4096, e.g. the 150 of the reference's 100-cell file at 25-cell
-    pads; extent=smooth widens it to the next compiled extent and records it."""
+    pads or 106 = 2 * 53 (no compiled split); extent=smooth widens an extent
+    without compiled kernels to the next compiled one and records it."""
+    import ose_amd
     sim = H.Simulation("nx=100 ny=101 nz=102 boundary=25 model=const nt=4").build()
     assert sim.model_dims(0)[9:12] == [150, 152, 152]
+    sim.close()
+    sim = H.Simulation("nx=56 ny=101 nz=102 boundary=25 model=const nt=4").build()
+    assert sim.model_dims(0)[9:12] == [106, 152, 152]
     assert sim.info()["models"][0]["extent_adjusted"] == []
     sim.close()
-    sim = H.Simulation("nx=100 ny=101 nz=102 boundary=25 model=const nt=4 extent=smooth").build()
+    sim = H.Simulation("nx=56 ny=101 nz=102 boundary=25 model=const nt=4 extent=smooth").build()
     d = sim.model_dims(0)
-    import ose_amd
-    assert d[9] == ose_amd.supported_extent(150) > 150 and d[4] == 25 + d[9] - 150
-    assert sim.info()["models"][0]["extent_adjusted"] == [{"axis": "x", "from": 150, "to": d[9]}]
+    assert ose_amd.extent_kind(106) == 2
+    assert d[9] == ose_amd.supported_extent(106) > 106 and d[4] == 25 + d[9] - 106 and d[10:12] == [152, 152]
+    assert sim.info()["models"][0]["extent_adjusted"] == [{"axis": "x", "from": 106, "to": d[9]}]
     sim.close()
     with pytest.raises(Exception):
         H.Simulation("nx=4090 ny=8 nz=8 boundary=4 model=const nt=4").build()
