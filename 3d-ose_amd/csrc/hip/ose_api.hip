@@ -2521,6 +2521,22 @@ int ose_set_boundary_net(ose_ctx* c, int model_id, const float* params, size_t n
     return OSE_OK;
 }
 
+int ose_trip_boundary_net(ose_ctx* c, int model_id)
+{
+    if (!c) return OSE_EINVAL;
+    if (model_id < 0 || model_id >= (int)c->models.size()) return c->fail(OSE_EINVAL, "ose_trip_boundary_net: bad model");
+    Model& m = *c->models[model_id];
+    if (!m.learned) return c->fail(OSE_ESTATE, "ose_trip_boundary_net: the model has no boundary network");
+    int rc;
+    if ((rc = set_device(c)) || (rc = finalize(c)) || (rc = ose_sync(c))) return rc;
+    if (!m.bb.tripped) return c->fail(OSE_ESTATE, "ose_trip_boundary_net: the model has no shots");
+    const int one = 1;
+    HIPCHK(c, hipMemcpy(m.bb.tripped, &one, sizeof(int), hipMemcpyHostToDevice));
+    m.learned_on = false;
+    drop_graph(c);
+    return OSE_OK;
+}
+
 int ose_boundary_status(ose_ctx* c, int model_id, int* active, int* tripped)
 {
     if (!c) return OSE_EINVAL;

@@ -741,6 +741,48 @@ int oseh_sim_snapshot(oseh_sim* s, const char* dir)
 // Checkpoint: ckpt.txt (step, shot count, per shot the padded extents and
 // receiver count), and per shot the wavefield (re, im: float32 [z][y][x])
 // and the traces recorded so far ([nrec][step] float32).
+// FNV-1a (64-bit) over the bytes that fix a run: every model's grid, pads,
+// velocity, order, coefficients and tapers, every shot's source cell and
+// receiver count, the wavelets, the spacing and the time step.  A checkpoint
+// records it and a restore refuses a checkpoint of any other configuration.
+static uint64_t run_fingerprint(const oseh_sim* s)
+{
+    uint64_t h = 1469598103934665603ULL;
+    auto mix = [&h](const void* p, size_t n) {
+        const unsigned char* c = static_cast<const unsigned char*>(p);
+        for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ULL;
+    };
+    for (const BuiltModel& m : s->models) {
+        const int dims[9] = {m.bl, m.br, m.bf, m.bb, m.bu, m.bd, m.nxb, m.nyb, m.nzb};
+        mix(dims, sizeof dims);
+        mix(m.v.data(), m.v.size() * sizeof(float));
+        mix(&m.K, sizeof m.K);
+        mix(m.coef.data(), m.coef.size() * sizeof(double));
+        mix(m.gx.data(), m.gx.size() * sizeof(float));
+        mix(m.gy.data(), m.gy.size() * sizeof(float));
+        mix(m.gz.data(), m.gz.size() * sizeof(float));
+    }
+    for (size_t k = 0; k < s->shots.size(); ++k) {
+        mix(s->shots[k].src, sizeof s->shots[k].src);
+        const size_t nrec = receiver_cells(s, k).size();
+        mix(&nrec, sizeof nrec);
+    }
+    mix(s->s.data(), s->s.size() * sizeof(float));
+    mix(s->h.data(), s->h.size() * sizeof(float));
+    const double dd[4] = {s->getd("dx", 10), s->getd("dy", 10), s->getd("dz", 10), s->getd("dt", 1e-3)};
+    mix(dd, sizeof dd);
+    return h;
+}
+
+static std::string ckpt_base(const std::string& dir, int step, size_t k)
+{
+    return dir + "/ckpt-s" + std::to_string(step) + "-shot" + std::to_string(k);
+}
+
+// Writes the state of every shot after `step` steps into files named for the
+// step, then replaces ckpt.txt through a rename (atomic within a
+// directory), then removes the files of the checkpoint it replaced: a crash
+// at any point leaves a ckpt.txt whose files are all present.
 int oseh_sim_checkpoint(oseh_sim* s, const char* dir)
 {
     if (!s || !dir) return OSE_EINVAL;
@@ -748,8 +790,18 @@ int oseh_sim_checkpoint(oseh_sim* s, const char* dir)
     int rc, step = 0;
     if ((rc = ose_get_step(s->ctx, &step))) return s->fail(rc, std::string("ose_get_step: ") + ose_last_error(s->ctx));
     const std::string d(dir);
+    int old_step = -1;
+    {
+        std::ifstream f(d + "/ckpt.txt");
+        std::string tag, key;
+        int version = 0;
+        if (f >> tag >> version >> key >> old_step && !(tag == "ose-checkpoint" && version == 2 && key == "step"))
+            old_step = -1;
+    }
     std::ostringstream meta;
-    meta << "ose-checkpoint 1\nstep " << step << "\nshots " << s->shots.size() << "\n";
+    char fp[32];
+    std::snprintf(fp, sizeof fp, "%016llx", (unsigned long long)run_fingerprint(s));
+    meta << "ose-checkpoint 2\nstep " << step << "\nfingerprint " << fp << "\nshots " << s->shots.size() << "\n";
     for (size_t k = 0; k < s->shots.size(); ++k) {
         const BuiltModel& m = s->models[s->shots[k].model];
         const size_t n = (size_t)m.nxb * m.nyb * m.nzb;
@@ -759,15 +811,31 @@ int oseh_sim_checkpoint(oseh_sim* s, const char* dir)
         if ((rc = ose_get_wavefield(s->ctx, (int)k, re.data(), im.data())) ||
             (step > 0 && (rc = ose_get_traces(s->ctx, (int)k, 0, step, tr.data()))))
             return s->fail(rc, std::string("checkpoint read-back: ") + ose_last_error(s->ctx));
-        const std::string base = d + "/ckpt-shot" + std::to_string(k);
+        const std::string base = ckpt_base(d, step, k);
         if ((rc = oseh_write_raw((base + "-re.bin").c_str(), re.data(), n)) ||
             (rc = oseh_write_raw((base + "-im.bin").c_str(), im.data(), n)) ||
             (rc = oseh_write_raw((base + "-traces.bin").c_str(), tr.data(), tr.size())))
             return s->fail(rc, "cannot write checkpoint " + base);
     }
-    std::ofstream f(d + "/ckpt.txt");
-    f << meta.str();
-    if (!f) return s->fail(OSE_EINVAL, "cannot write " + d + "/ckpt.txt");
+    // the learned pad's state per model: whether its guard has tripped
+    for (size_t mi = 0; mi < s->models.size(); ++mi) {
+        int active = 0, tripped = 0;
+        if ((rc = ose_boundary_status(s->ctx, s->models[mi].id, &active, &tripped)))
+            return s->fail(rc, std::string("ose_boundary_status: ") + ose_last_error(s->ctx));
+        meta << "bnet " << mi << " " << active << " " << tripped << "\n";
+    }
+    {
+        std::ofstream f(d + "/ckpt.txt.tmp");
+        f << meta.str();
+        f.flush();
+        if (!f) return s->fail(OSE_EINVAL, "cannot write " + d + "/ckpt.txt.tmp");
+    }
+    if (std::rename((d + "/ckpt.txt.tmp").c_str(), (d + "/ckpt.txt").c_str()))
+        return s->fail(OSE_EINVAL, "cannot replace " + d + "/ckpt.txt");
+    if (old_step >= 0 && old_step != step)
+        for (size_t k = 0; k < s->shots.size(); ++k)
+            for (const char* suf : {"-re.bin", "-im.bin", "-traces.bin"})
+                std::remove((ckpt_base(d, old_step, k) + suf).c_str());
     return OSE_OK;
 }
 
@@ -785,12 +853,17 @@ int oseh_sim_restore(oseh_sim* s, const char* dir)
     if (!s->ctx) return s->fail(OSE_ESTATE, "not set up");
     const std::string d(dir);
     std::ifstream f(d + "/ckpt.txt");
-    std::string tag, key;
+    std::string tag, key, fp;
     int version = 0, step = -1;
     size_t nshots = 0;
-    if (!(f >> tag >> version) || tag != "ose-checkpoint" || version != 1 || !(f >> key >> step) || key != "step" ||
-        !(f >> key >> nshots) || key != "shots")
+    if (!(f >> tag >> version) || tag != "ose-checkpoint" || version != 2 || !(f >> key >> step) || key != "step" ||
+        !(f >> key >> fp) || key != "fingerprint" || !(f >> key >> nshots) || key != "shots")
         return s->fail(OSE_EINVAL, "not a checkpoint: " + d + "/ckpt.txt");
+    char want[32];
+    std::snprintf(want, sizeof want, "%016llx", (unsigned long long)run_fingerprint(s));
+    if (fp != want)
+        return s->fail(OSE_EINVAL, "checkpoint " + d + " was written by another configuration (model, order, "
+                                   "coefficients, tapers, sources, receivers, wavelets or spacing differ)");
     if (nshots != s->shots.size() || step < 0 || step > s->geti("nt", 300))
         return s->fail(OSE_EINVAL, "checkpoint does not match this configuration");
     int rc;
@@ -803,13 +876,26 @@ int oseh_sim_restore(oseh_sim* s, const char* dir)
             return s->fail(OSE_EINVAL, "checkpoint shot " + std::to_string(k) + " does not match this configuration");
         const size_t n = (size_t)nxb * nyb * nzb;
         std::vector<float> re(n), im(n), tr((size_t)nrec * step);
-        const std::string base = d + "/ckpt-shot" + std::to_string(k);
+        const std::string base = ckpt_base(d, step, k);
         if (read_floats(base + "-re.bin", re) || read_floats(base + "-im.bin", im) ||
             read_floats(base + "-traces.bin", tr))
             return s->fail(OSE_EINVAL, "cannot read checkpoint " + base);
         if ((rc = ose_set_wavefield(s->ctx, (int)k, re.data(), im.data())) ||
             (step > 0 && (rc = ose_set_traces(s->ctx, (int)k, 0, step, tr.data()))))
             return s->fail(rc, std::string("checkpoint upload: ") + ose_last_error(s->ctx));
+    }
+    // a learned pad whose guard had tripped continues on the plain taper
+    for (size_t mi = 0; mi < s->models.size(); ++mi) {
+        size_t idx = 0;
+        int active = 0, tripped = 0;
+        if (!(f >> key >> idx >> active >> tripped) || key != "bnet" || idx != mi)
+            return s->fail(OSE_EINVAL, "checkpoint lacks the boundary-network state of model " + std::to_string(mi));
+        int now_active = 0, now_tripped = 0;
+        if ((rc = ose_boundary_status(s->ctx, s->models[mi].id, &now_active, &now_tripped)))
+            return s->fail(rc, std::string("ose_boundary_status: ") + ose_last_error(s->ctx));
+        if (tripped && !now_tripped && (now_active || active) &&
+            (rc = ose_trip_boundary_net(s->ctx, s->models[mi].id)))
+            return s->fail(rc, std::string("ose_trip_boundary_net: ") + ose_last_error(s->ctx));
     }
     if ((rc = ose_set_step(s->ctx, step))) return s->fail(rc, std::string("ose_set_step: ") + ose_last_error(s->ctx));
     s->steps_done = step;

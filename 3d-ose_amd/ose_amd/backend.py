@@ -183,6 +183,10 @@ class Context:
         self._check(self._L.ose_boundary_status(self._h, model, ctypes.byref(a), ctypes.byref(t)))
         return dict(active=bool(a.value), tripped=bool(t.value))
 
+    def trip_boundary_net(self, model):
+        """Force the network's fallback to the plain taper (as a guard trip)."""
+        self._check(self._L.ose_trip_boundary_net(self._h, model))
+
     def set_options(self, use_graph=True, profile=False):
         self._check(self._L.ose_set_options(self._h, int(use_graph), int(profile)))
 

@@ -338,6 +338,12 @@ class Simulation:
             raise OseError(rc, self._hip.ose_last_error(self.ctx).decode())
         return dict(active=bool(a.value), tripped=bool(t.value))
 
+    def trip_boundary_net(self, model):
+        """Force a model's learned pad into its tripped state (plain taper)."""
+        rc = self._hip.ose_trip_boundary_net(self.ctx, model)
+        if rc:
+            raise OseError(rc, self._hip.ose_last_error(self.ctx).decode())
+
     # model/shot data (for the oracle)
     def nmodels(self):
         return self._L.oseh_sim_nmodels(self._h)

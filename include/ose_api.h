@@ -276,6 +276,12 @@ int ose_set_boundary_net(ose_ctx* ctx, int model_id, const float* params, size_t
                          int replace_taper);
 /* active: the network is in use; tripped: its guard has fired.             */
 int ose_boundary_status(ose_ctx* ctx, int model_id, int* active, int* tripped);
+/* Put a model's network into the tripped state from the host, as if its
+ * guard had fired: the plain taper from the next step on, status tripped.
+ * A run resumed from a checkpoint taken after a trip restores it this way,
+ * so it continues as the uninterrupted run would.  No counterpart in the
+ * reference.                                                              */
+int ose_trip_boundary_net(ose_ctx* ctx, int model_id);
 
 /* Training object of the network on one padded grid (single sample at a
  * time; BF16 forward and backward on MFMA, FP32 Adam).  Fields u, t are

@@ -80,6 +80,69 @@ def test_checkpoint_resume_equals_straight_run(tmp_path):
     assert np.abs(im_b - im_a).max() <= 1e-6 * np.abs(im_a).max()
 
 
+def test_checkpoint_refuses_another_configuration(tmp_path):
+    """A checkpoint records a fingerprint of the run (velocity, order,
+    coefficients, tapers, sources, receivers, wavelets, spacing): a
+    simulation with the same grid and shots but another velocity or wavelet
+    refuses it; checkpoints replace their files (only the latest remain)."""
+    from ose_amd import host as H
+    print("checkpoint fingerprint", flush=True)
+    a = H.Simulation("preset=ref nt=20").setup()
+    try:
+        a.run(5)
+        a.checkpoint(tmp_path)
+        a.run(5)
+        a.checkpoint(tmp_path)
+    finally:
+        a.close()
+    names = sorted(p.name for p in tmp_path.iterdir())
+    assert names == ["ckpt-s10-shot0-im.bin", "ckpt-s10-shot0-re.bin", "ckpt-s10-shot0-traces.bin", "ckpt.txt"]
+    for other in ("v1=4200", "fm=30", "korder=12"):
+        b = H.Simulation(f"preset=ref nt=20 {other}").setup()
+        try:
+            with pytest.raises(Exception, match="another configuration"):
+                b.restore(tmp_path)
+        finally:
+            b.close()
+    c = H.Simulation("preset=ref nt=20").setup()
+    try:
+        c.restore(tmp_path)
+        assert c.info()["nt"] == 20
+    finally:
+        c.close()
+
+
+def test_resume_after_a_tripped_guard_equals_straight_run(tmp_path):
+    """The learned pad's guard trips (forced from the host) at step 8; a
+    checkpoint at step 12 records it, and the resumed run continues on the
+    plain taper exactly as the uninterrupted run does."""
+    from ose_amd import boundary
+    from ose_amd import host as H
+    print("resume after a tripped guard", flush=True)
+    cfg = f"nx=64 ny=64 nz=64 boundary=8 model=layered8 seed=3 nt=24 sx=32 sy=32 sz=2 bnet={boundary.WEIGHTS}"
+    a = H.Simulation(cfg).setup()
+    b = H.Simulation(cfg).setup()
+    try:
+        a.run(8)
+        assert a.boundary_status(0) == {"active": True, "tripped": False}
+        a.trip_boundary_net(0)
+        a.run(4)
+        a.checkpoint(tmp_path)
+        a.run(12)
+        tr_a, (re_a, im_a) = a.traces(0, 0, 24), a.wavefield(0)
+        assert a.boundary_status(0) == {"active": False, "tripped": True}
+        b.restore(tmp_path)
+        assert b.boundary_status(0) == {"active": False, "tripped": True}
+        b.run(12)
+        tr_b, (re_b, im_b) = b.traces(0, 0, 24), b.wavefield(0)
+    finally:
+        a.close()
+        b.close()
+    assert np.abs(tr_a).max() > 0
+    assert np.array_equal(tr_b, tr_a)
+    assert np.array_equal(re_b, re_a) and np.array_equal(im_b, im_a)
+
+
 def test_driver_checkpoint_and_restore(tmp_path):
     nt = 40
     ck = tmp_path / "ck"
