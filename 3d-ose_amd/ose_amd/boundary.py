@@ -83,14 +83,25 @@ def extent(n, pad_before, pad_after_min):
     return nb - n - pad_before
 
 
-def make_pair_grids(vi, thin, wide, vmax=None, taper="ref", thin_taper=None):
+def make_pair_grids(vi, thin, wide, vmax=None, taper="ref", thin_taper=None, wide_taper=None):
     n = vi.shape[::-1]
     ta = [extent(n[a], thin, thin) for a in range(3)]
     wa = [extent(n[a], wide, wide) for a in range(3)]
     vm = float(vi.max()) if vmax is None else vmax
     thin_g = Grid(vi, (thin,) * 3, tuple(ta), taper=thin_taper or taper, vmax=vm)
-    wide_g = Grid(vi, (wide,) * 3, tuple(wa), taper=taper, vmax=vm)
+    wide_g = Grid(vi, (wide,) * 3, tuple(wa), taper=wide_taper or taper, vmax=vm)
     return thin_g, wide_g
+
+
+def wide_setup(wide, n, thin, nt):
+    """(pad, taper) of the runs that supply the training targets: wide > 0,
+    a sponge of that many cells with the reference taper; wide = 0, a
+    taper-free domain wide enough that nothing leaving the thin window
+    returns to it within nt steps (the targets are then free of any
+    absorber's reflections)."""
+    if wide > 0:
+        return wide, None
+    return unreflected_pad(n, nt) + thin, "none"
 
 
 def _wavelet(nt, fm, dt, delay=40):
@@ -101,10 +112,11 @@ def _wavelet(nt, fm, dt, delay=40):
 def generate_pairs(net, seeds, n=64, thin=8, wide=64, nt=400, t0=120, every=8, fm=20.0, replace=True, log=print):
     """Add (u_thin(t+1), u_wide(t+1)|window) pairs of the given seeded models to net."""
     count = 0
+    wide, wtaper = wide_setup(wide, n, thin, nt)
     for seed in seeds:
         vi = H.model("train", n, n, n, seed=seed)
         # a network that replaces the taper of the thin pad learns from untapered steps
-        thin_g, wide_g = make_pair_grids(vi, thin, wide, thin_taper="none" if replace else None)
+        thin_g, wide_g = make_pair_grids(vi, thin, wide, thin_taper="none" if replace else None, wide_taper=wtaper)
         rng = np.random.default_rng(seed)
         src = (int(rng.integers(n // 4, 3 * n // 4)), int(rng.integers(n // 4, 3 * n // 4)),
                int(rng.integers(0, n // 3)))
@@ -137,9 +149,10 @@ def generate_rollout_pairs(net, params, seeds, n=64, thin=8, wide=64, nt=400, t0
                            fm=20.0, replace=True, guard=4.0, log=print):
     """Pairs whose inputs come from thin runs with the network in the loop."""
     count = 0
+    wide, wtaper = wide_setup(wide, n, thin, nt + every)
     for seed in seeds:
         vi = H.model("train", n, n, n, seed=seed)
-        thin_g, wide_g = make_pair_grids(vi, thin, wide, thin_taper="none" if replace else None)
+        thin_g, wide_g = make_pair_grids(vi, thin, wide, thin_taper="none" if replace else None, wide_taper=wtaper)
         rng = np.random.default_rng(seed)
         src = (int(rng.integers(n // 4, 3 * n // 4)), int(rng.integers(n // 4, 3 * n // 4)),
                int(rng.integers(0, n // 3)))
@@ -257,13 +270,80 @@ def evaluate(params, seeds, n=64, thin=8, wide=64, nt=400, fm=20.0, model_kind="
     return rows
 
 
+def unreflected_pad(n, nt, dt=1e-3, d=10.0, vmax=4500.0, margin=16):
+    """Pad width of a taper-free reference run on which nothing that leaves
+    the interior returns to the surface receivers within nt steps.  The
+    periodic domain's nearest image of a source at the surface centre lies
+    n + 2W away along x or y, i.e. n/2 + 2W from the farthest receiver; a
+    wave covers at most vmax nt dt / d cells, so W > (that - n/2) / 2; the
+    margin covers the spectral operator's tail ahead of the wavefront."""
+    reach = vmax * nt * dt / d
+    return max(8, int(np.ceil((reach - n / 2) / 2)) + margin)
+
+
+def run_traces(g, src, s, h, nt, params=None, guard=4.0, replace=True):
+    """Surface seismogram of one run on grid g (and the learned pad's status)."""
+    with Context(0) as c:
+        m = g.add_to(c)
+        sid = c.add_shot(m, tuple(g.origin[a] + src[a] for a in range(3)), s, h)
+        if params is not None:
+            c.set_boundary_net(m, params, guard, replace_taper=replace)
+        t0 = time.time()
+        c.run(nt)
+        c.sync()
+        dt = time.time() - t0
+        tr = c.traces(sid)
+        st = c.boundary_status(m) if params is not None else None
+    return tr, st, dt
+
+
+def compare_pads(seeds, boundaries, n=64, nt=400, fm=20.0, model_kind="train", log=None, check=False):
+    """Seismogram rel-L2 of thin boundaries against an unreflected reference.
+
+    boundaries: list of (name, width, params) -- params None for the plain
+    reference taper of that width (generate_att), else the learned pad's
+    network (replacing the taper).  The reference is a taper-free run on a
+    domain wide enough (unreflected_pad) that nothing leaving the interior
+    returns to the receivers within the window; with check=True it is
+    compared with one 32 cells wider, which must agree to FP32 rounding."""
+    rows = []
+    for seed in seeds:
+        vi = H.model(model_kind, n, n, n, seed=seed)
+        vm = float(vi.max())
+        wref = unreflected_pad(n, nt, vmax=vm)
+        s, h = _wavelet(nt, fm, 1e-3)
+        src = (n // 2, n // 2, 2)
+        row = {"seed": int(seed), "vmax": vm, "ref_pad": wref}
+        refs = {}
+        for w in ((wref, wref + 32) if check else (wref,)):
+            g = Grid(vi, (w,) * 3, tuple(extent(vi.shape[::-1][a], w, w) for a in range(3)), taper="none", vmax=vm)
+            refs[w], _, t = run_traces(g, src, s, h, nt)
+            row.setdefault("ref_padded", list(g.shape[::-1]))
+        ref = refs[wref]
+        nref = np.linalg.norm(ref)
+        if check:
+            row["ref_vs_wider"] = float(np.linalg.norm(refs[wref + 32] - ref) / nref)
+        for name, w, p in boundaries:
+            g = Grid(vi, (w,) * 3, tuple(extent(vi.shape[::-1][a], w, w) for a in range(3)),
+                     taper="none" if p is not None else "ref", vmax=vm)
+            tr, st, t = run_traces(g, src, s, h, nt, params=p)
+            row[name] = float(np.linalg.norm(tr - ref) / nref)
+            if st is not None:
+                row[name + "_status"] = st
+        rows.append(row)
+        if log:
+            log(json.dumps(row))
+    return rows
+
+
 def main():
     import argparse
     ap = argparse.ArgumentParser(description="train / evaluate the learned thin-pad network")
-    ap.add_argument("cmd", choices=["train", "eval"])
+    ap.add_argument("cmd", choices=["train", "eval", "compare"])
     ap.add_argument("--n", type=int, default=64)
     ap.add_argument("--thin", type=int, default=8)
-    ap.add_argument("--wide", type=int, default=64)
+    ap.add_argument("--wide", type=int, default=64,
+                    help="train: sponge width of the target runs; 0: taper-free targets on an unreflected domain")
     ap.add_argument("--models", type=int, default=16)
     ap.add_argument("--iters", type=int, default=4000)
     ap.add_argument("--lr", type=float, default=2e-3)
@@ -277,7 +357,20 @@ def main():
     ap.add_argument("--plain-only", action="store_true", help="eval: wide sponge and thin taper only")
     ap.add_argument("--init", default=None, help="train: parameters to continue from")
     ap.add_argument("--seed0", type=int, default=1000, help="train: first model seed")
+    ap.add_argument("--plain", default="8,16,24,32,40", help="compare: plain taper widths")
+    ap.add_argument("--learned", default="", help="compare: width=weights.npy pairs, comma separated")
+    ap.add_argument("--check-ref", action="store_true", help="compare: also run a 32-cell wider reference")
     args = ap.parse_args()
+    if args.cmd == "compare":
+        a, b = (int(x) for x in args.seeds.split(":"))
+        bnd = [(f"plain{w}", int(w), None) for w in args.plain.split(",") if w]
+        for item in (x for x in args.learned.split(",") if x):
+            w, path = item.split("=")
+            stem = os.path.splitext(os.path.basename(path))[0]
+            bnd.append((f"learned{w}_{stem}", int(w), np.load(path)))
+        compare_pads(range(a, b), bnd, n=args.n, nt=args.nt, log=lambda m: print(m, flush=True),
+                     check=args.check_ref)
+        return
     if args.cmd == "train":
         init = np.load(args.init) if args.init else None
         train(range(args.seed0, args.seed0 + args.models), n=args.n, thin=args.thin, wide=args.wide, iters=args.iters,

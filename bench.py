@@ -146,13 +146,37 @@ def set_streams(sim, n):
         raise RuntimeError(f"ose_set_streams({n}) failed: {rc}")
 
 
+LEARNED_PROFILE = os.path.join(ROOT, "profiles", "learned_pad_unreflected_64cube_400steps.txt")
+
+
+def learned_accuracy():
+    """Medians of the committed comparison against the unreflected
+    reference: seismogram rel-L2 per boundary, and which plain taper width
+    the 8-cell learned pad matches (the widest plain taper whose median
+    error is not below the learned pad's)."""
+    import numpy as np
+    rows = [json.loads(line) for line in open(LEARNED_PROFILE) if line.startswith("{")]
+    keys = [k for k in rows[0] if k.startswith(("plain", "learned")) and not k.endswith("_status")]
+    med = {k: float(np.median([r[k] for r in rows])) for k in keys}
+    l8 = med["learned8_bnet_thin8"]
+    widths = sorted(int(k[5:]) for k in med if k.startswith("plain"))
+    match = max((w for w in widths if med[f"plain{w}"] >= l8), default=None)
+    return med, match, len(rows)
+
+
 def learned_leg(params, steps, warmup, plain_value, weights, streams):
     """The same workload with an 8-cell pad: plain thin taper, then the
-    learned pad (network in the step, replacing the taper)."""
+    learned pad (network in the step, replacing the taper); and the plain
+    taper of the width whose accuracy the learned pad matches, for the speed
+    ratio at equal accuracy."""
     from ose_amd.host import Simulation
+    med, match, nseeds = learned_accuracy()
     res = {}
-    for name, extra in (("thin_taper", ""), ("learned", f"bnet={weights}")):
-        sim = Simulation(f"{params} boundary=8 nt={warmup + steps + 1} {extra}")
+    legs = [("thin_taper", "boundary=8", ""), ("learned", "boundary=8", f"bnet={weights}")]
+    if match and match != 8:
+        legs.append((f"plain{match}", f"boundary={match}", ""))
+    for name, pad, extra in legs:
+        sim = Simulation(f"{params} {pad} nt={warmup + steps + 1} {extra}")
         sim.setup()
         set_streams(sim, streams)
         info = sim.info()
@@ -163,6 +187,7 @@ def learned_leg(params, steps, warmup, plain_value, weights, streams):
         print(f"[bench] {name}: {ms / steps:.1f} ms/step", file=sys.stderr, flush=True)
         sim.close()
     lr = res["learned"]
+    eq = res.get(f"plain{match}")
     return {
         "value": lr["value"],
         "unit": "Gcell*steps/s",
@@ -172,13 +197,77 @@ def learned_leg(params, steps, warmup, plain_value, weights, streams):
         "thin_taper_value": res["thin_taper"]["value"],
         "network_ms_per_step": lr["ms_per_step"] - res["thin_taper"]["ms_per_step"],
         "weights": os.path.relpath(weights, ROOT),
-        "accuracy": ("not at the rel-L2 1e-3 bar: on held-out 256^3 models over 2000 steps the trained 8-cell "
-                     "pad leaves 5.4-9.9x less reflected interior energy than the 8-cell taper; seismogram rel-L2 "
-                     "against a 64-cell sponge 0.52-0.53 (taper 0.60-0.72); profiles/learned_pad_eval_256cube_2000steps.txt"),
+        "equal_accuracy": {
+            "matches_plain_taper_cells": match,
+            "plain_value": eq["value"] if eq else None,
+            "plain_padded": eq["padded"] if eq else None,
+            "speed_ratio": lr["value"] / eq["value"] if eq else None,
+            "reaches_plain40": med["learned8_bnet_thin8"] <= med["plain40"],
+        },
+        "accuracy": (f"seismogram rel-L2 against an unreflected reference, median over {nseeds} held-out 64^3 models "
+                     f"x 400 steps: learned 8-cell {med['learned8_bnet_thin8']:.3f}, plain taper "
+                     + ", ".join(f"{k[5:]}-cell {v:.3f}" for k, v in sorted(med.items(), key=lambda kv: -kv[1])
+                                 if k.startswith("plain"))
+                     + f"; the learned pad matches the {match}-cell taper, no learned pad up to 16 cells reaches "
+                       f"the 40-cell one (the 1e-3 bar is not met by any boundary); "
+                     + os.path.relpath(LEARNED_PROFILE, ROOT)),
     }
 
 
 SNAP_CASES = (("f32", 3, False), ("bf16", 2, True), ("q16", 2, True), ("q8", 2, True))
+
+
+def accuracy_leg(nsteps=20):
+    """Accuracy measured in the same run, for the metric's "at seismogram
+    rel-L2 <= 1e-3 vs FP64":
+      fp64: the 256^3 configuration (336^3 padded, K = 12, 40-cell taper)
+            over the first 100 steps against the committed FP64 oracle
+            seismogram (tests/golden/c1_oracle.npz, every 8th receiver);
+      closed_form: one 864^3 shot of the 768^3 workload's geometry (K = 11,
+            the same kernels and launch shapes), constant v and no taper,
+            from a sum of four Fourier modes; after nsteps each mode must
+            carry exp(-i n dt v |k|) (tests/_planewave.py: amplitude and
+            phase errors and the energy outside the modes, with bounds)."""
+    import numpy as np
+    from ose_amd.host import Simulation
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _planewave as pw
+    out = {}
+    g = np.load(os.path.join(ROOT, "tests", "golden", "c1_oracle.npz"))
+    nt = int(g["nt"])
+    sim = Simulation(f"preset=c1 nt={nt}").setup()
+    try:
+        sim.run(nt)
+        tr = sim.traces(0, 0, nt)
+    finally:
+        sim.close()
+    ref = g["traces"].astype(np.float64)
+    out["fp64"] = {"workload": f"256^3 (336^3 padded), K = {int(g['K'])}, {nt} steps, every 8th surface receiver",
+                   "rel_l2": float(np.linalg.norm(tr[g["idx"]] - ref) / np.linalg.norm(ref))}
+    v = 4500.0
+    sim = Simulation(f"preset=c4 model=const v0={v} taper=none amp=0 nt={nsteps} max_shots=1").setup()
+    try:
+        m = sim.model(0)
+        shape = tuple(m["v"].shape)
+        nz, ny, nx = shape
+        modes = [(3, 5, 7), (nx // 3, ny // 2 - 1, nz // 5), (nx - 40, 31, nz // 2 - 3), (nx // 2, ny // 2, nz // 2)]
+        amps = [1.0, 0.5 - 0.25j, 0.3j, 0.2]
+        re, im = pw.modes_field(shape, modes, amps)
+        sim.set_wavefield(0, re, im)
+        del re, im
+        sim.run(nsteps)
+        re, im = sim.wavefield(0)
+        z = m["R"] * 1e-3
+        import scipy.special as sp
+        tail = 2.0 * sum(abs(sp.jv(k, z)) for k in range(m["K"] + 1, m["K"] + 40))
+        r = pw.check_modes(re, im, modes, amps, nsteps, 1e-3, shape, (10.0, 10.0, 10.0), v, tail, z)
+    finally:
+        sim.close()
+    out["closed_form"] = dict({"workload": f"one {nx}^3 shot, K = {m['K']}, {nsteps} steps"},
+                              **{k: float(x) for k, x in r.items()})
+    out["within_bounds"] = bool(out["fp64"]["rel_l2"] <= 1e-4 and r["amp"] < r["amp_bound"] and
+                                r["phase"] < r["phase_bound"] and r["rest"] < r["rest_bound"])
+    return out
 
 
 def snapshot_leg(params, steps, every, warmup=2):
@@ -259,6 +348,46 @@ def snapshot_leg(params, steps, every, warmup=2):
     return {"params": params, "models": info["models"], "shots": shots, "steps": steps, "runs": out}
 
 
+def shape_sweep(params, reps=5):
+    """Every compiled strided launch shape of the workload's first model,
+    pinned in turn: mean HIP-event duration of one launch of forward-y,
+    plain forward-z, the z-symbol pass and inverse-y over all shots
+    (ose_pass_probe), with the algorithmic 16 B/cell rate."""
+    import ctypes
+    from ose_amd import _lib
+    from ose_amd.host import Simulation
+    L = _lib.hip()
+    sim = Simulation(f"{params} nt=4").setup()
+    ctx = sim.ctx
+    info = sim.info()
+    st, xs = (ctypes.c_int * 16)(), (ctypes.c_int * 16)()
+    ns, nx = ctypes.c_int(16), ctypes.c_int(16)
+    L.ose_launch_shapes(ctx, 0, st, ctypes.byref(ns), xs, ctypes.byref(nx))
+    codes = list(st)[:ns.value]
+    sim.run(1)
+    base = (ctypes.c_int * 4)()
+    L.ose_get_tuning(ctx, 0, base)
+    cells = info["models"][0]["shots"] * 1.0
+    for n in info["models"][0]["padded"]:
+        cells *= n
+    rows = []
+    for code in codes:
+        t = (ctypes.c_int * 4)(code, code, code, base[3])
+        if L.ose_set_tuning(ctx, 0, t):
+            raise RuntimeError(L.ose_last_error(ctx).decode())
+        row = {"code": code}
+        for which, name in ((0, "fwd_y"), (1, "fwd_z"), (2, "z_symbol"), (3, "inv_y")):
+            ms = ctypes.c_double()
+            if L.ose_pass_probe(ctx, 0, which, reps, ctypes.byref(ms)):
+                raise RuntimeError(L.ose_last_error(ctx).decode())
+            row[name] = {"ms": round(ms.value, 3), "TBs": round(16.0 * cells / (ms.value * 1e-3) / 1e12, 3)}
+        rows.append(row)
+        print(json.dumps(row), flush=True)
+    sim.close()
+    return {"params": params, "padded": info["models"][0]["padded"], "shots": info["models"][0]["shots"],
+            "autotuned": list(base), "rows": rows}
+
+
 def main():
     if len(sys.argv) == 3 and sys.argv[1] == "--rocprof-summary":
         print(rocprof_summary(sys.argv[2]))
@@ -275,11 +404,14 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the 8-cell learned-pad leg")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the accuracy leg")
     ap.add_argument("--streams", type=int, default=2, choices=(1, 2),
                     help="2: the step's shots run as two concurrent lanes (default); 1: one stream")
     ap.add_argument("--snapshots", type=int, default=0, metavar="EVERY",
                     help="instead of the throughput line: the cost of streaming snapshots every EVERY steps "
                          "(one JSON line per encoding, then a summary)")
+    ap.add_argument("--shape-sweep", action="store_true",
+                    help="instead of the throughput line: every strided launch shape's pass durations")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run one plain device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -293,6 +425,9 @@ def main():
     params, workload = CONFIGS[args.config]
     if args.snapshots > 0:
         print(json.dumps(snapshot_leg(f"{params} {args.extra}", args.steps, args.snapshots)))
+        return
+    if args.shape_sweep:
+        print(json.dumps(shape_sweep(f"{params} {args.extra}")))
         return
     prof_steps = max(2, min(args.steps, 4))
     nt = args.warmup + args.steps + prof_steps
@@ -340,6 +475,10 @@ def main():
     m0 = info["models"][0]
     tune = tuning(sim, len(info["models"]))
     sim.close()
+    accuracy = None
+    if not args.no_accuracy:
+        accuracy = accuracy_leg()
+        print(f"[bench] accuracy: {json.dumps(accuracy)}", file=sys.stderr, flush=True)
     learned = None
     if not args.no_learned and args.config in ("c4", "c1"):
         from ose_amd.boundary import WEIGHTS
@@ -373,6 +512,7 @@ def main():
             "launch_shapes": tune,
         },
         "learned": learned,
+        "accuracy": accuracy,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -94,6 +94,25 @@ def test_reference_as_shipped_matches_fp64_oracle():
     assert abs(ratio - 1.0) < 1e-4
 
 
+def test_learned_pad_against_unreflected_reference():
+    """Held-out seeds 2000-2002 (64^3, 400 steps) against a taper-free
+    reference on a domain wide enough that nothing returns within the window
+    (checked against one 32 cells wider): the committed 8-cell learned pad
+    is more accurate than the reference's taper at 8 and at 16 cells.  (No
+    learned pad up to 16 cells reaches the 40-cell taper:
+    profiles/learned_pad_unreflected_64cube_400steps.txt.)"""
+    from ose_amd import boundary
+    p = np.load(boundary.WEIGHTS)
+    print("learned pad vs unreflected reference", flush=True)
+    rows = boundary.compare_pads([2000, 2001, 2002], [("plain8", 8, None), ("plain16", 16, None), ("learned8", 8, p)],
+                                 check=True)
+    for r in rows:
+        print(r, flush=True)
+        assert r["ref_vs_wider"] < 1e-4
+        assert not r["learned8_status"]["tripped"]
+        assert r["learned8"] < r["plain8"] and r["learned8"] < r["plain16"]
+
+
 def test_trained_thin_pad_beats_plain_taper():
     """Held-out seeded model (64^3 interior): the committed 8-cell learned
     pad against the 8-cell reference taper, both measured against a 64-cell
