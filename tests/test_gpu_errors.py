@@ -45,11 +45,11 @@ def test_model_arguments_are_checked():
         assert _err(lambda: c.add_model(case.v, rec_z=case.nz, **kw))[0] == OSE_EINVAL
         bad = dict(kw, interior=(case.nxb, case.ny, case.nz))          # origin + interior beyond the grid
         assert _err(lambda: c.add_model(case.v, **bad))[0] == OSE_EINVAL
-        # 17 is not a compiled FFT extent
-        v17 = np.full((48, 32, 17), 2000.0, dtype=np.float32)
-        kw17 = dict(kw, interior=(5, case.ny, case.nz), taper=(np.ones(17), case.gy, case.gz))
-        code, msg = _err(lambda: c.add_model(v17, **kw17))
-        assert code == OSE_EUNSUPPORTED and "not compiled" in msg
+        # every extent in [2, 4096] runs (17: the runtime-length path); 4098 does not
+        vbig = np.full((48, 32, 4098), 2000.0, dtype=np.float32)
+        kwbig = dict(kw, interior=(5, case.ny, case.nz), taper=(np.ones(4098), case.gy, case.gz))
+        code, msg = _err(lambda: c.add_model(vbig, **kwbig))
+        assert code == OSE_EUNSUPPORTED and "outside [2, 4096]" in msg
         # the context is still usable after the failures
         m = case.add_to(c)
         s = ricker(10, 30.0, case.dt)
@@ -60,7 +60,7 @@ def test_model_arguments_are_checked():
 
 def test_shot_and_state_errors():
     import ose_amd
-    from ose_amd._lib import OSE_EINVAL, OSE_ENUMERIC, OSE_ESTATE
+    from ose_amd._lib import OSE_EINVAL, OSE_ENUMERIC, OSE_ESTATE, OSE_EUNSUPPORTED
     print("shot and state errors", flush=True)
     case = _case()
     s = ricker(20, 30.0, case.dt)
@@ -81,7 +81,7 @@ def test_shot_and_state_errors():
         re, im = c.wavefield(sid)
         re[1, 2, 3] = np.nan
         assert _err(lambda: c.set_wavefield(sid, re, im))[0] == OSE_ENUMERIC
-        assert _err(lambda: c.fft3d(np.zeros((16, 16, 17), np.complex64)))[0] != 0
+        assert _err(lambda: c.fft3d(np.zeros((16, 16, 1), np.complex64)))[0] == OSE_EUNSUPPORTED   # extent 1
         assert _err(lambda: c.set_streams(3))[0] == OSE_EINVAL
         c.set_streams(1)
         c.run(1)

@@ -107,8 +107,8 @@ def test_runtime_length_path_equals_compiled_path(K):
                                               ("3d_number-100-100-100-3", (100, 100, 100), (150, 150, 150))])
 def test_reference_input_files_on_their_own_grid(name, dims, padded, tmp_path):
     """The reference's two other velocity files at its 25-cell padding,
-    extent=exact, so x (and for the 100^3 file every axis) is 150 cells on
-    the runtime-length path: 60 steps against the FP64 oracle
+    extent=exact, so x (and for the 100^3 file every axis) is 150 cells
+    (compiled in a single launch shape): 60 steps against the FP64 oracle
     (tests/golden/gen_refinput_oracle.py; the files are stored there
     byte for byte)."""
     import os
