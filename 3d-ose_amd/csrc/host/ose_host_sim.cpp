@@ -221,6 +221,13 @@ int axis_pads(oseh_sim* s, const char* axis, int n, int b, int& before, int& aft
     }
     if (!ose_extent_is_supported(nb))
         return s->fail(OSE_EUNSUPPORTED, "padded extent " + std::to_string(nb) + " is outside [2, 4096]");
+    if (ose_extent_kind(nb) == 2) {
+        const int t = ose_supported_extent(nb);
+        std::fprintf(stderr, "ose: the %s padded extent %d has no compiled transform kernels and runs on the "
+                             "slower runtime-length path%s%s\n",
+                     axis, nb, t > 0 ? "; extent=smooth would pad it to " : "",
+                     t > 0 ? std::to_string(t).c_str() : "");
+    }
     return OSE_OK;
 }
 
