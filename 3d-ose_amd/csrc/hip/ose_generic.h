@@ -20,18 +20,27 @@
 // so one LDS buffer suffices.  Radices 2, 3, 4, 5, 7, 8, 11, 13 and 16 use the
 // straight-line register DFTs of dft_gen.h; a larger prime factor R is a
 // direct stage: each output sums its R inputs with table twiddles (O(R) per
-// element, no recurrence, so no drift in FP32).
+// element, no recurrence, so no drift in FP32).  The twiddles come from a
+// copy of the root table at the head of the workgroup's LDS.
 #pragma once
 #include "ose_kernels.h"
 
 namespace ose {
 
 constexpr int GEN_MAXV = 16;   // values per thread per stage
+constexpr int GEN_G = 4;       // outputs per thread and LDS read in a direct stage
+
+// The workgroup's LDS: the axis's root table w^j (j < N) first, then the
+// pencils or lines.  Declared once here so that the stage functions, which
+// are not inlined (each keeps its own register allocation, so the kernel
+// needs the largest stage's registers, not the sum of all), address it as
+// LDS.
+extern __shared__ float2 gen_lds[];
 
 template <int SIGN>
-__device__ __forceinline__ float2 gtw(const float2* __restrict__ tw, int e)
+__device__ __forceinline__ float2 gtw(int e)
 {
-    const float2 w = tw[e];
+    const float2 w = gen_lds[e];
     return SIGN < 0 ? w : make_float2(w.x, -w.y);
 }
 
@@ -42,10 +51,10 @@ struct GenAt {
 };
 
 template <int R, int SIGN>
-__device__ __forceinline__ void gen_stage(float2* L, GenAt at, int N, int TP, int p, const float2* __restrict__ tw,
-                                          int t)
+__device__ __noinline__ void gen_stage(GenAt at, int N, int TP, int p, int t)
 {
     constexpr int NB = GEN_MAXV / R;   // butterflies per thread
+    float2* L = gen_lds;
     const int M = N / R;
     const int tws = N / (p * R);
     float2 v[NB][R];
@@ -57,7 +66,7 @@ __device__ __forceinline__ void gen_stage(float2* L, GenAt at, int N, int TP, in
 #pragma unroll
             for (int s = 0; s < R; ++s) v[b][s] = L[at(i + s * M)];
 #pragma unroll
-            for (int s = 1; s < R; ++s) v[b][s] = cmul(v[b][s], gtw<SIGN>(tw, s * k * tws));
+            for (int s = 1; s < R; ++s) v[b][s] = cmul(v[b][s], gtw<SIGN>(s * k * tws));
             rdft<R, SIGN>(v[b]);
         }
     }
@@ -75,39 +84,58 @@ __device__ __forceinline__ void gen_stage(float2* L, GenAt at, int N, int TP, in
     __syncthreads();
 }
 
-// a stage of prime radix R > 16: output o = j + s p (j = (i - k) R + k) is
-// sum_q x[i + q N/R] w^(q e), e = k N/(p R) + s N/R < N
+// A stage of prime radix R > 13: output j + s p (j = (i - k) R + k) of
+// butterfly i is sum_q x[i + q N/R] w^(q e_s), e_s = k N/(p R) + s N/R < N.
+// A thread takes groups of GEN_G outputs of one butterfly (consecutive s),
+// so each input read from LDS serves GEN_G products; the exponents advance
+// by e_s mod N (table lookups, no recurrence).
 template <int SIGN>
-__device__ __forceinline__ void gen_direct(float2* L, GenAt at, int N, int TP, int p, int R,
-                                           const float2* __restrict__ tw, int t)
+__device__ __noinline__ void gen_direct(GenAt at, int N, int TP, int p, int R, int t)
 {
+    constexpr int NB = GEN_MAXV / GEN_G;   // groups per thread
+    float2* L = gen_lds;
     const int M = N / R;
     const int tws = N / (p * R);
-    float2 v[GEN_MAXV];
+    const int ng = (R + GEN_G - 1) / GEN_G;
+    float2 v[NB][GEN_G];
 #pragma unroll
-    for (int b = 0; b < GEN_MAXV; ++b) {
-        const int o = t + b * TP;
-        if (o < N) {
-            const int s = (o / p) % R, k = o % p, i = (o / (p * R)) * p + k;
-            const int e = k * tws + s * M;
-            float2 acc = make_float2(0.f, 0.f);
-            int idx = 0;
+    for (int b = 0; b < NB; ++b) {
+        const int u = t + b * TP;
+        if (u < M * ng) {
+            const int i = u / ng, s0 = (u - i * ng) * GEN_G;
+            const int k = i % p;
+            int e[GEN_G], idx[GEN_G];
+#pragma unroll
+            for (int g = 0; g < GEN_G; ++g) {
+                e[g] = k * tws + min(s0 + g, R - 1) * M;
+                idx[g] = 0;
+                v[b][g] = make_float2(0.f, 0.f);
+            }
             for (int q = 0; q < R; ++q) {
                 const float2 x = L[at(i + q * M)];
-                const float2 w = gtw<SIGN>(tw, idx);
-                acc.x += x.x * w.x - x.y * w.y;
-                acc.y += x.x * w.y + x.y * w.x;
-                idx += e;
-                if (idx >= N) idx -= N;
+#pragma unroll
+                for (int g = 0; g < GEN_G; ++g) {
+                    const float2 w = gtw<SIGN>(idx[g]);
+                    v[b][g].x += x.x * w.x - x.y * w.y;
+                    v[b][g].y += x.x * w.y + x.y * w.x;
+                    idx[g] += e[g];
+                    if (idx[g] >= N) idx[g] -= N;
+                }
             }
-            v[b] = acc;
         }
     }
     __syncthreads();
 #pragma unroll
-    for (int b = 0; b < GEN_MAXV; ++b) {
-        const int o = t + b * TP;
-        if (o < N) L[at(o)] = v[b];
+    for (int b = 0; b < NB; ++b) {
+        const int u = t + b * TP;
+        if (u < M * ng) {
+            const int i = u / ng, s0 = (u - i * ng) * GEN_G;
+            const int k = i % p;
+            const int j = (i - k) * R + k;
+#pragma unroll
+            for (int g = 0; g < GEN_G; ++g)
+                if (s0 + g < R) L[at(j + (s0 + g) * p)] = v[b][g];
+        }
     }
     __syncthreads();
 }
@@ -115,84 +143,93 @@ __device__ __forceinline__ void gen_direct(float2* L, GenAt at, int N, int TP, i
 // unnormalised DFT (SIGN -1 forward, +1 inverse) of one pencil in LDS, natural
 // order in and out; every thread of the workgroup calls it (barriers inside)
 template <int SIGN>
-__device__ __forceinline__ void gen_fft(float2* L, GenAt at, const GenPlan& g, const float2* __restrict__ tw, int t)
+__device__ __forceinline__ void gen_fft(GenAt at, const GenPlan& g, int t)
 {
     int p = 1;
     for (int f = 0; f < g.nf; ++f) {
         const int R = g.r[f];
         switch (R) {
-        case 2: gen_stage<2, SIGN>(L, at, g.N, g.TP, p, tw, t); break;
-        case 3: gen_stage<3, SIGN>(L, at, g.N, g.TP, p, tw, t); break;
-        case 4: gen_stage<4, SIGN>(L, at, g.N, g.TP, p, tw, t); break;
-        case 5: gen_stage<5, SIGN>(L, at, g.N, g.TP, p, tw, t); break;
-        case 7: gen_stage<7, SIGN>(L, at, g.N, g.TP, p, tw, t); break;
-        case 8: gen_stage<8, SIGN>(L, at, g.N, g.TP, p, tw, t); break;
-        case 11: gen_stage<11, SIGN>(L, at, g.N, g.TP, p, tw, t); break;
-        case 13: gen_stage<13, SIGN>(L, at, g.N, g.TP, p, tw, t); break;
-        case 16: gen_stage<16, SIGN>(L, at, g.N, g.TP, p, tw, t); break;
-        default: gen_direct<SIGN>(L, at, g.N, g.TP, p, R, tw, t); break;
+        case 2: gen_stage<2, SIGN>(at, g.N, g.TP, p, t); break;
+        case 3: gen_stage<3, SIGN>(at, g.N, g.TP, p, t); break;
+        case 4: gen_stage<4, SIGN>(at, g.N, g.TP, p, t); break;
+        case 5: gen_stage<5, SIGN>(at, g.N, g.TP, p, t); break;
+        case 7: gen_stage<7, SIGN>(at, g.N, g.TP, p, t); break;
+        case 8: gen_stage<8, SIGN>(at, g.N, g.TP, p, t); break;
+        case 11: gen_stage<11, SIGN>(at, g.N, g.TP, p, t); break;
+        case 13: gen_stage<13, SIGN>(at, g.N, g.TP, p, t); break;
+        case 16: gen_stage<16, SIGN>(at, g.N, g.TP, p, t); break;
+        default: gen_direct<SIGN>(at, g.N, g.TP, p, R, t); break;
         }
         p *= R;
     }
 }
 
+// the axis's root table into the head of the LDS (the data's load waits for it)
+__device__ __forceinline__ void gen_load_table(const float2* __restrict__ tw, int N)
+{
+    for (int j = threadIdx.x; j < N; j += blockDim.x) gen_lds[j] = tw[j];
+}
+
 // the runtime-length strided pass: W = T x columns per workgroup, TP threads
-// per column; LDS rows of T + 1 elements
+// per column; LDS rows of T + 1 elements after the root table
 struct GenStrided {
-    int T, col, t, x0, xc;
+    int T, col, t, x0, xc, N;
     long base;
     __device__ __forceinline__ GenStrided(const StridedArgs& a, const StridedTile& g)
     {
         T = a.g.W;
+        N = a.g.N;
         col = threadIdx.x % T;
         t = threadIdx.x / T;
         x0 = g.bx * T;
         xc = min(col, a.ncol - 1 - x0);   // columns past the edge load the last one, store nothing
         base = (long)g.bz * a.shot_stride + (long)g.by * a.os + x0 + xc;
     }
-    __device__ __forceinline__ GenAt at() const { return GenAt{T + 1, col}; }
-    __device__ __forceinline__ void load(const StridedArgs& a, float2* L) const
+    __device__ __forceinline__ GenAt at() const { return GenAt{T + 1, N + col}; }
+    __device__ __forceinline__ void load(const StridedArgs& a) const
     {
-        for (int r = t; r < a.g.N; r += a.g.TP) L[r * (T + 1) + col] = a.in[base + (long)r * a.es];
+        gen_load_table(a.tw, N);
+        const GenAt e = at();
+        for (int r = t; r < N; r += a.g.TP) gen_lds[e(r)] = a.in[base + (long)r * a.es];
         __syncthreads();
     }
-    __device__ __forceinline__ void store(const StridedArgs& a, const float2* L) const
+    __device__ __forceinline__ void store(const StridedArgs& a) const
     {
         if (col != xc) return;
-        for (int r = t; r < a.g.N; r += a.g.TP) a.out[base + (long)r * a.es] = L[r * (T + 1) + col];
+        const GenAt e = at();
+        for (int r = t; r < N; r += a.g.TP) a.out[base + (long)r * a.es] = gen_lds[e(r)];
     }
 };
 
 template <int SIGN>
 __global__ __launch_bounds__(512) void k_gen_strided(StridedArgs a)
 {
-    extern __shared__ float2 lds[];
     const StridedTile g(a);
     const GenStrided s(a, g);
-    s.load(a, lds);
-    gen_fft<SIGN>(lds, s.at(), a.g, a.tw, s.t);
-    s.store(a, lds);
+    s.load(a);
+    gen_fft<SIGN>(s.at(), a.g, s.t);
+    s.store(a);
 }
 
 // z pass: forward-z, * |k| / N (scale_kxkz_3D, ose_3d_kernel.cu:101-126, and
 // the 1/N of scal_vr_3d, :355), inverse-z, as k_zsym
 __global__ __launch_bounds__(512) void k_gen_zsym(StridedArgs a)
 {
-    extern __shared__ float2 lds[];
     const StridedTile g(a);
     const GenStrided s(a, g);
-    s.load(a, lds);
-    gen_fft<-1>(lds, s.at(), a.g, a.tw, s.t);
+    s.load(a);
+    gen_fft<-1>(s.at(), a.g, s.t);
     const float kxy = a.kx2[s.x0 + s.xc] + a.ky2[g.by];
-    for (int r = s.t; r < a.g.N; r += a.g.TP) {
-        float2& v = lds[r * (s.T + 1) + s.col];
+    const GenAt e = s.at();
+    for (int r = s.t; r < s.N; r += a.g.TP) {
+        float2& v = gen_lds[e(r)];
         const float m = __builtin_amdgcn_sqrtf(kxy + a.kz2[r]) * a.invN;
         v.x *= m;
         v.y *= m;
     }
     __syncthreads();
-    gen_fft<1>(lds, s.at(), a.g, a.tw, s.t);
-    s.store(a, lds);
+    gen_fft<1>(s.at(), a.g, s.t);
+    s.store(a);
 }
 
 // x pass: W = P lines per workgroup, TP threads per line; the same modes,
@@ -200,7 +237,6 @@ __global__ __launch_bounds__(512) void k_gen_zsym(StridedArgs a)
 template <int MODE>
 __global__ __launch_bounds__(512) void k_gen_xpass(XArgs a)
 {
-    extern __shared__ float2 lds[];
     const int N = a.g.N, P = a.g.W, TP = a.g.TP;
     const int l = threadIdx.x / TP;
     const int t = threadIdx.x % TP;
@@ -212,20 +248,22 @@ __global__ __launch_bounds__(512) void k_gen_xpass(XArgs a)
     const int z = okl ? (int)(line / a.ny) : 0;
     const long lbase = (long)blockIdx.y * a.shot_stride + line * N;
     const long vbase = line * N;
-    float2* L = lds + (long)l * N;
-    const GenAt at{1, l * N};
+    // the root table, then P lines
+    float2* L = gen_lds + N + (long)l * N;
+    const GenAt at{1, N + l * N};
     const float2 zero = make_float2(0.f, 0.f);
+    gen_load_table(a.tw, N);   // the forward table (twf) holds the same roots
 
     if (MODE == XM_FWD || MODE == XM_INV) {
         const float2* src = MODE == XM_FWD ? a.A : a.Xprev;
         for (int x = t; x < N; x += TP) L[x] = okl ? src[lbase + x] : zero;
         __syncthreads();
         if (MODE == XM_FWD) {
-            gen_fft<-1>(lds, at, a.g, a.twf, t);
+            gen_fft<-1>(at, a.g, t);
             if (okl)
                 for (int x = t; x < N; x += TP) a.Xout[lbase + x] = L[x];
         } else {
-            gen_fft<1>(lds, at, a.g, a.tw, t);
+            gen_fft<1>(at, a.g, t);
             if (okl)
                 for (int x = t; x < N; x += TP) a.A[lbase + x] = make_float2(L[x].x * a.inv_nx, L[x].y * a.inv_nx);
         }
@@ -235,7 +273,7 @@ __global__ __launch_bounds__(512) void k_gen_xpass(XArgs a)
     // L W_j / (v/R) in physical x (unnormalised inverse; the symbol carried 1/N)
     for (int x = t; x < N; x += TP) L[x] = okl ? a.S[lbase + x] : zero;
     __syncthreads();
-    gen_fft<1>(lds, at, a.g, a.tw, t);
+    gen_fft<1>(at, a.g, t);
 
     if (okl) {
         const float gyz = a.gy[y] * a.gz[z];
@@ -291,7 +329,7 @@ __global__ __launch_bounds__(512) void k_gen_xpass(XArgs a)
     }
     if (MODE == XM_LASTA) return;
     __syncthreads();
-    gen_fft<-1>(lds, at, a.g, a.twf, t);
+    gen_fft<-1>(at, a.g, t);
     if (okl)
         for (int x = t; x < N; x += TP) a.Xout[lbase + x] = L[x];
 }

@@ -38,15 +38,19 @@ bool plan_radices(int n, GenPlan& g)
 }
 
 // values each thread holds in a stage of radix r: r per butterfly, at most
-// GEN_MAXV; direct stages hold one output each
+// GEN_MAXV; a direct stage's groups of GEN_G outputs, at most GEN_MAXV
 int threads_per_pencil(const GenPlan& g)
 {
-    int tp = (g.N + GEN_MAXV - 1) / GEN_MAXV;
+    int tp = 1;
     for (int f = 0; f < g.nf; ++f) {
-        const int r = g.r[f];
-        if (r > 16) continue;
-        const int nb = GEN_MAXV / r, m = g.N / r;
-        tp = std::max(tp, (m + nb - 1) / nb);
+        const int r = g.r[f], m = g.N / r;
+        if (r > 16) {
+            const int groups = m * ((r + GEN_G - 1) / GEN_G), per = GEN_MAXV / GEN_G;
+            tp = std::max(tp, (groups + per - 1) / per);
+        } else {
+            const int nb = GEN_MAXV / r;
+            tp = std::max(tp, (m + nb - 1) / nb);
+        }
     }
     return tp;
 }
@@ -77,17 +81,18 @@ const FftEntry* generic_entry(int N)
     if (!plan_radices(N, g)) return nullptr;
     g.TP = threads_per_pencil(g);
     if (g.TP > kThreadsMax) return nullptr;
-    // strided: the widest T (x columns, rows of T + 1 elements in LDS) that fits
+    // LDS: the root table (N entries), then the pencils or lines
+    // strided: the widest T (x columns, rows of T + 1 elements) that fits
     int T = 0;
     for (int t : {16, 8, 4, 2, 1})
-        if (t * g.TP <= kThreadsMax && sizeof(float2) * (size_t)N * (t + 1) <= kLdsMax) {
+        if (t * g.TP <= kThreadsMax && sizeof(float2) * (size_t)N * (t + 2) <= kLdsMax) {
             T = t;
             break;
         }
     // x pass: up to 8 lines per workgroup
     int P = 0;
     for (int p : {8, 4, 2, 1})
-        if (p * g.TP <= kThreadsMax && sizeof(float2) * (size_t)p * N <= kLdsMax) {
+        if (p * g.TP <= kThreadsMax && sizeof(float2) * (size_t)(p + 1) * N <= kLdsMax) {
             P = p;
             break;
         }
@@ -105,7 +110,7 @@ const FftEntry* generic_entry(int N)
     k.T = T;
     k.split = 0;
     k.threads = T * g.TP;
-    k.lds = sizeof(float2) * (size_t)N * (T + 1);
+    k.lds = sizeof(float2) * (size_t)N * (T + 2);
     k.gp = &pl[0];
     k.fn = (const void*)&k_gen_strided<-1>;
     e->yf[0] = k;
@@ -119,7 +124,7 @@ const FftEntry* generic_entry(int N)
     x.split = 0;
     x.twb = 0;
     x.threads = P * g.TP;
-    x.lds = sizeof(float2) * (size_t)P * N;
+    x.lds = sizeof(float2) * (size_t)(P + 1) * N;
     x.gp = &pl[1];
     x.fn[XM_ORDER0] = (const void*)&k_gen_xpass<XM_ORDER0>;
     x.fn[XM_MID] = (const void*)&k_gen_xpass<XM_MID>;
