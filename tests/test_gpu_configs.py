@@ -113,6 +113,28 @@ def test_learned_pad_against_unreflected_reference():
         assert r["learned8"] < r["plain8"] and r["learned8"] < r["plain16"]
 
 
+def test_unreflected_reference_matches_fp64_oracle():
+    """The unreflected reference of the learned-pad comparison (seed 2018,
+    taper-free 200^3 domain) over its first steps against the FP64 oracle
+    (tests/golden/gen_unreflected_oracle.py, every second receiver)."""
+    import sys
+    from ose_amd import boundary as B
+    path = os.path.join(GOLD, "unreflected_oracle.npz")
+    g = np.load(path)
+    sys.path.insert(0, GOLD)
+    import gen_unreflected_oracle as U
+    grid = U.grid()
+    assert list(grid.shape[::-1]) == list(g["padded"]) and grid.K == int(g["K"])
+    nt = int(g["nt"])
+    print(f"unreflected reference, seed {int(g['seed'])}, {nt} steps", flush=True)
+    s, h = B._wavelet(U.WINDOW, 20.0, 1e-3)
+    tr, _, _ = B.run_traces(grid, (U.N // 2, U.N // 2, 2), s, h, nt)
+    err = rel(tr[g["idx"], :nt], g["traces"])
+    ratio = np.linalg.norm(tr[:, :nt].astype(np.float64)) / float(g["norm_full"])
+    print(f"  rel-L2 {err:.3g}, norm ratio {ratio:.7f}", flush=True)
+    assert err < 1e-4 and abs(ratio - 1.0) < 1e-4
+
+
 def test_trained_thin_pad_beats_plain_taper():
     """Held-out seeded model (64^3 interior): the committed 8-cell learned
     pad against the 8-cell reference taper, both measured against a 64-cell
