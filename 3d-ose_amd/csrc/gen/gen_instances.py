@@ -189,9 +189,11 @@ def main():
     factors = sorted({f for (a, b) in allsizes.values() for f in (a, b)} | set(GENERIC_RADICES))
     subprocess.check_call([sys.executable, os.path.join(HERE, "gen_dft.py"),
                            os.path.join(out, "dft_gen.h")] + [str(f) for f in factors])
+    # the registry always lists every extent; `only` limits which instance
+    # sources are (re)written
     with open(os.path.join(out, "ose_sizes.inc"), "w") as f:
         f.write("// Generated by csrc/gen/gen_instances.py -- do not edit.\n")
-        for n in sorted(sizes):
+        for n in sorted(allsizes):
             f.write(f"OSE_SIZE({n})\n")
     for n in sorted(sizes):
         n1, n2 = sizes[n]

@@ -233,7 +233,7 @@ __global__ __launch_bounds__(512) void k_gen_zsym(StridedArgs a)
 }
 
 // x pass: W = P lines per workgroup, TP threads per line; the same modes,
-// arguments and per-element update as k_xpass
+// arguments and per-element update (x_update) as k_xpass
 template <int MODE>
 __global__ __launch_bounds__(512) void k_gen_xpass(XArgs a)
 {
@@ -293,33 +293,7 @@ __global__ __launch_bounds__(512) void k_gen_xpass(XArgs a)
         for (int x = t; x < N; x += TP) {
             const float2 u = L[x];
             const float vr = a.vr[vbase + x];
-            const float2 lw = make_float2(vr * u.x, vr * u.y);
-            float2 wn;
-            if (MODE == XM_ORDER0 || MODE == XM_ORDER0N) {
-                // W1 = -i L W0
-                wn = make_float2(lw.y, -lw.x);
-                if (MODE == XM_ORDER0) {
-                    const float2 wp = win[x];
-                    acc[x] = make_float2(a.c0 * wp.x + a.c1 * wn.x, a.c0 * wp.y + a.c1 * wn.y);
-                }
-            } else {
-                // W_{j+1} = W_{j-1} - 2i L W_j
-                const float2 wp = win[x];
-                wn = make_float2(wp.x + 2.f * lw.y, wp.y - 2.f * lw.x);
-                if (MODE != XM_SKIP) {
-                    const float gg = gyz * a.gx[x];
-                    const float gp = gpow(gg, a.eb) * a.cp, gn = gpow(gg, a.ec) * a.c1;
-                    float2 an = make_float2(gp * wp.x + gn * wn.x, gp * wp.y + gn * wn.y);
-                    if (MODE != XM_MIDF) {
-                        const float2 ao = acc[x];
-                        const float ga = gpow(gg, a.ea);
-                        an.x += ga * ao.x;
-                        an.y += ga * ao.y;
-                    }
-                    if (MODE == XM_MID || MODE == XM_MIDF) acc[x] = an;
-                    if (MODE == XM_LAST || MODE == XM_LASTA) wn = an;   // the new state u
-                }
-            }
+            const float2 wn = x_update<MODE>(a, make_float2(vr * u.x, vr * u.y), x, gyz, win, acc);
             L[x] = wn;
             a.Wout[lbase + x] = wn;
             if (rec && x >= a.rec_x0 && x < a.rec_x0 + a.rec_nx) tr[x - a.rec_x0] = wn.y;

@@ -502,6 +502,47 @@ __device__ __forceinline__ float gpow(float g, int e)
     return (e & 2) ? g1 * (g * g) : g1;
 }
 
+// Pass 4's update of one element x of a line: lw = L W_j there (v/R times the
+// unnormalised inverse x transform; the symbol carried 1/N).  Returns W_{j+1}
+// (W_1 in ORDER0*, the new state u in LAST*) and updates the accumulator
+// where the mode touches it; win and acc point at the line.  Win and Wout
+// may be the same buffer: the caller reads win[x] here before it stores
+// W_{j+1} at x, and no other thread touches x.  Shared by k_xpass and the
+// runtime-length k_gen_xpass.
+template <int MODE>
+__device__ __forceinline__ float2 x_update(const XArgs& a, float2 lw, int x, float gyz, const float2* win,
+                                           float2* __restrict__ acc)
+{
+    float2 wn;
+    if (MODE == XM_ORDER0 || MODE == XM_ORDER0N) {
+        // W1 = -i L W0
+        wn = make_float2(lw.y, -lw.x);
+        if (MODE == XM_ORDER0) {
+            const float2 wp = win[x];
+            acc[x] = make_float2(a.c0 * wp.x + a.c1 * wn.x, a.c0 * wp.y + a.c1 * wn.y);
+        }
+    } else {
+        // W_{j+1} = W_{j-1} - 2i L W_j
+        const float2 wp = win[x];
+        wn = make_float2(wp.x + 2.f * lw.y, wp.y - 2.f * lw.x);
+        if (MODE != XM_SKIP) {
+            const float g = gyz * a.gx[x];
+            const float gp = gpow(g, a.eb) * a.cp, gn = gpow(g, a.ec) * a.c1;
+            float2 an = make_float2(gp * wp.x + gn * wn.x, gp * wp.y + gn * wn.y);
+            if (MODE != XM_MIDF) {
+                const float2 ao = acc[x];
+                const float ga = gpow(g, a.ea);
+                an.x += ga * ao.x;
+                an.y += ga * ao.y;
+            }
+            if (MODE == XM_MID || MODE == XM_MIDF) acc[x] = an;
+            if (MODE == XM_LAST || MODE == XM_LASTA)
+                wn = an;   // the new state u; its x transform is X_0 of the next step
+        }
+    }
+    return wn;
+}
+
 template <int N1, int N2, int P, int MODE, int SPLIT, int TWB>
 __global__ __launch_bounds__((XCfg<N1, N2, P, SPLIT>::THREADS))
 void k_xpass(XArgs a)
@@ -565,35 +606,7 @@ void k_xpass(XArgs a)
         for (int n1 = 0; n1 < N1; ++n1) {
             const int x = N2 * n1 + t;
             const float vr = a.vr[vbase + x];
-            const float2 lw = make_float2(vr * u[n1].x, vr * u[n1].y);
-            float2 wn;
-            if (MODE == XM_ORDER0 || MODE == XM_ORDER0N) {
-                // W1 = -i L W0
-                wn = make_float2(lw.y, -lw.x);
-                if (MODE == XM_ORDER0) {
-                    const float2 wp = win[x];
-                    acc[x] = make_float2(a.c0 * wp.x + a.c1 * wn.x, a.c0 * wp.y + a.c1 * wn.y);
-                }
-            } else {
-                // W_{j+1} = W_{j-1} - 2i L W_j
-                const float2 wp = win[x];
-                wn = make_float2(wp.x + 2.f * lw.y, wp.y - 2.f * lw.x);
-                if (MODE != XM_SKIP) {
-                    const float g = gyz * a.gx[x];
-                    const float gp = gpow(g, a.eb) * a.cp, gn = gpow(g, a.ec) * a.c1;
-                    float2 an = make_float2(gp * wp.x + gn * wn.x, gp * wp.y + gn * wn.y);
-                    if (MODE != XM_MIDF) {
-                        const float2 ao = acc[x];
-                        const float ga = gpow(g, a.ea);
-                        an.x += ga * ao.x;
-                        an.y += ga * ao.y;
-                    }
-                    if (MODE == XM_MID || MODE == XM_MIDF) acc[x] = an;
-                    if (MODE == XM_LAST || MODE == XM_LASTA)
-                        wn = an;   // the new state u; its x transform is X_0 of the next step
-                }
-            }
-            u[n1] = wn;
+            u[n1] = x_update<MODE>(a, make_float2(vr * u[n1].x, vr * u[n1].y), x, gyz, win, acc);
         }
         float2* wout = a.Wout + lbase;
 #pragma unroll
