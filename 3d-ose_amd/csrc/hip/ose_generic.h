@@ -19,8 +19,9 @@
 // Each stage reads everything it needs into registers, waits, writes, waits,
 // so one LDS buffer suffices.  Radices 2, 3, 4, 5, 7, 8, 11, 13 and 16 use the
 // straight-line register DFTs of dft_gen.h; a larger prime factor R is a
-// direct stage: each output sums its R inputs with table twiddles (O(R) per
-// element, no recurrence, so no drift in FP32).  The twiddles come from a
+// direct stage: output pairs (s, R - s) sum the R inputs' symmetric and
+// antisymmetric combinations with table cosines and sines (O(R) per element,
+// no recurrence, so no drift in FP32).  The twiddles come from a
 // copy of the root table at the head of the workgroup's LDS.
 #pragma once
 #include "ose_kernels.h"
@@ -28,7 +29,6 @@
 namespace ose {
 
 constexpr int GEN_MAXV = 16;   // values per thread per stage
-constexpr int GEN_G = 4;       // outputs per thread and LDS read in a direct stage
 
 // The workgroup's LDS: the axis's root table w^j (j < N) first, then the
 // pencils or lines.  Declared once here so that the stage functions, which
@@ -84,43 +84,60 @@ __device__ __noinline__ void gen_stage(GenAt at, int N, int TP, int p, int t)
     __syncthreads();
 }
 
-// A stage of prime radix R > 13: output j + s p (j = (i - k) R + k) of
-// butterfly i is sum_q x[i + q N/R] w^(q e_s), e_s = k N/(p R) + s N/R < N.
-// A thread takes groups of GEN_G outputs of one butterfly (consecutive s),
-// so each input read from LDS serves GEN_G products; the exponents advance
-// by e_s mod N (table lookups, no recurrence).
+// A stage of prime radix R > 13.  Butterfly i (k = i mod p) twiddles its
+// inputs, y_q = x[i + q N/R] w^(q k N/(p R)), and forms the R-point DFT in
+// the pair-symmetric way: with a_q = y_q + y_(R-q), b_q = y_q - y_(R-q)
+// (q = 1..h, h = (R-1)/2) and the angle t = 2 pi q s / R,
+//   X_0 = y_0 + sum_q a_q,   X_s, X_(R-s) = y_0 + sum_q a_q cos t -/+ i S sum_q b_q sin t
+// (S = +1 forward, -1 inverse), written to j + s p, j = (i - k) R + k.  A
+// thread takes a group of GEN_GP output pairs of one butterfly (s = 0 stands
+// for X_0 alone), so each a_q, b_q serves GEN_GP pairs; cos and sin come
+// from the root table at (q s mod R) N/R (no recurrence).  Half the products
+// and a quarter of the table reads of the plain sum over all R outputs.
+constexpr int GEN_GP = 4;   // output pairs per thread group
+
 template <int SIGN>
 __device__ __noinline__ void gen_direct(GenAt at, int N, int TP, int p, int R, int t)
 {
-    constexpr int NB = GEN_MAXV / GEN_G;   // groups per thread
+    constexpr int NB = GEN_MAXV / (2 * GEN_GP);   // groups per thread
     float2* L = gen_lds;
     const int M = N / R;
     const int tws = N / (p * R);
-    const int ng = (R + GEN_G - 1) / GEN_G;
-    float2 v[NB][GEN_G];
+    const int h = (R - 1) / 2;
+    const int ng = (h + GEN_GP) / GEN_GP;          // groups over s = 0..h
+    float2 v[NB][2 * GEN_GP];
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
         const int u = t + b * TP;
         if (u < M * ng) {
-            const int i = u / ng, s0 = (u - i * ng) * GEN_G;
+            const int i = u / ng, s0 = (u - i * ng) * GEN_GP;
             const int k = i % p;
-            int e[GEN_G], idx[GEN_G];
+            float2 A[GEN_GP], B[GEN_GP];
 #pragma unroll
-            for (int g = 0; g < GEN_G; ++g) {
-                e[g] = k * tws + min(s0 + g, R - 1) * M;
-                idx[g] = 0;
-                v[b][g] = make_float2(0.f, 0.f);
-            }
-            for (int q = 0; q < R; ++q) {
-                const float2 x = L[at(i + q * M)];
+            for (int g = 0; g < GEN_GP; ++g) A[g] = B[g] = make_float2(0.f, 0.f);
+            const float2 y0 = L[at(i)];
+            for (int q = 1; q <= h; ++q) {
+                const float2 y1 = cmul(L[at(i + q * M)], gtw<SIGN>(q * k * tws));
+                const float2 y2 = cmul(L[at(i + (R - q) * M)], gtw<SIGN>((R - q) * k * tws));
+                const float2 aq = make_float2(y1.x + y2.x, y1.y + y2.y);
+                const float2 bq = make_float2(y1.x - y2.x, y1.y - y2.y);
 #pragma unroll
-                for (int g = 0; g < GEN_G; ++g) {
-                    const float2 w = gtw<SIGN>(idx[g]);
-                    v[b][g].x += x.x * w.x - x.y * w.y;
-                    v[b][g].y += x.x * w.y + x.y * w.x;
-                    idx[g] += e[g];
-                    if (idx[g] >= N) idx[g] -= N;
+                for (int g = 0; g < GEN_GP; ++g) {
+                    const int s = min(s0 + g, h);
+                    const float2 w = gen_lds[((q * s) % R) * M];   // (cos t, -sin t)
+                    A[g].x += aq.x * w.x;
+                    A[g].y += aq.y * w.x;
+                    B[g].x -= bq.x * w.y;
+                    B[g].y -= bq.y * w.y;
                 }
+            }
+#pragma unroll
+            for (int g = 0; g < GEN_GP; ++g) {
+                // i S B = (-S B.y, S B.x)
+                const float bx = SIGN < 0 ? -B[g].y : B[g].y, by = SIGN < 0 ? B[g].x : -B[g].x;
+                const float2 c = make_float2(y0.x + A[g].x, y0.y + A[g].y);
+                v[b][2 * g] = make_float2(c.x - bx, c.y - by);       // X_s
+                v[b][2 * g + 1] = make_float2(c.x + bx, c.y + by);   // X_(R-s)
             }
         }
     }
@@ -129,12 +146,19 @@ __device__ __noinline__ void gen_direct(GenAt at, int N, int TP, int p, int R, i
     for (int b = 0; b < NB; ++b) {
         const int u = t + b * TP;
         if (u < M * ng) {
-            const int i = u / ng, s0 = (u - i * ng) * GEN_G;
+            const int i = u / ng, s0 = (u - i * ng) * GEN_GP;
             const int k = i % p;
             const int j = (i - k) * R + k;
 #pragma unroll
-            for (int g = 0; g < GEN_G; ++g)
-                if (s0 + g < R) L[at(j + (s0 + g) * p)] = v[b][g];
+            for (int g = 0; g < GEN_GP; ++g) {
+                const int s = s0 + g;
+                if (s == 0) {
+                    L[at(j)] = v[b][0];   // X_0 (its pair partner is unused)
+                } else if (s <= h) {
+                    L[at(j + s * p)] = v[b][2 * g];
+                    L[at(j + (R - s) * p)] = v[b][2 * g + 1];
+                }
+            }
         }
     }
     __syncthreads();

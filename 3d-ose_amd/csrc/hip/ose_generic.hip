@@ -38,14 +38,14 @@ bool plan_radices(int n, GenPlan& g)
 }
 
 // values each thread holds in a stage of radix r: r per butterfly, at most
-// GEN_MAXV; a direct stage's groups of GEN_G outputs, at most GEN_MAXV
+// GEN_MAXV; a direct stage's groups of GEN_GP output pairs, at most GEN_MAXV
 int threads_per_pencil(const GenPlan& g)
 {
     int tp = 1;
     for (int f = 0; f < g.nf; ++f) {
         const int r = g.r[f], m = g.N / r;
         if (r > 16) {
-            const int groups = m * ((r + GEN_G - 1) / GEN_G), per = GEN_MAXV / GEN_G;
+            const int groups = m * (((r - 1) / 2 + GEN_GP) / GEN_GP), per = GEN_MAXV / (2 * GEN_GP);
             tp = std::max(tp, (groups + per - 1) / per);
         } else {
             const int nb = GEN_MAXV / r;
