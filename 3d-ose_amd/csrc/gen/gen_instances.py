@@ -152,9 +152,36 @@ XPASS = """    {{
         e.xk[e.nx++] = k;
     }}
 """
+ZCHUNK = """    {{
+        constexpr int T = {T};
+        using ZC = ZChunkCfg<N1, N2, T>;
+        StridedKern k;
+        k.T = T; k.split = 0; k.threads = ZC::THREADS; k.lds = ZC::LDS_BYTES;
+        k.fn = (const void*)&k_zsym_chunk<N1, N2, T>;
+        for (int m : {{{MS}}}) {{
+            k.chunk = m;
+            e.zs[e.ns] = k;
+            e.yf[e.ns] = StridedKern();
+            e.yi[e.ns] = StridedKern();
+            ++e.ns;
+        }}
+    }}
+"""
 TAIL = """}}
 }}  // namespace ose
 """
+
+# chunked z pass (k_zsym_chunk): T = 16 columns, whole-complex exchange plus
+# the twiddle rows and |k_z|^2 in LDS, one workgroup per CU; compiled for the
+# autotuned extents from 256 up, where the z pass streams from HBM
+ZCHUNK_T = 16
+ZCHUNK_MS = (2, 3, 4, 6)
+
+
+def zchunk_fits(n1, n2):
+    n = n1 * n2
+    lds = 8 * (n * (ZCHUNK_T + 1) + 2 * n) + 4 * n
+    return n >= 256 and n1 <= n2 and ZCHUNK_T * n2 <= 1024 and lds <= LDS_MAX
 
 
 def lean_variants(n1, n2):
@@ -173,6 +200,8 @@ def instance_source(n, n1, n2, lean=False):
     strided, xs = lean_variants(n1, n2) if lean else (strided_variants(n1, n2), x_variants(n1, n2))
     for t, split in strided:
         src += STRIDED.format(T=t, SPLIT=split)
+    if not lean and zchunk_fits(n1, n2):
+        src += ZCHUNK.format(T=ZCHUNK_T, MS=", ".join(str(m) for m in ZCHUNK_MS))
     for p, split, twb in xs:
         src += XPASS.format(P=p, SPLIT=split, TWB=twb)
     return src + TAIL.format()

@@ -580,6 +580,23 @@ int collect(ose_ctx* c)
     return OSE_OK;
 }
 
+// The chunked z pass indexes tiles in 32 bits with the exact magic division
+// of StridedTile: < 2^24 tiles, extents < 2^16.
+bool chunk_usable(const StridedKern& k, int nx, int ny, int nshot)
+{
+    if (k.chunk <= 0) return true;
+    const double gx = (nx + k.T - 1) / k.T;
+    return gx * ny * nshot < 16777216.0 && gx < 65536 && ny < 65536;
+}
+
+// launch-shape code of a strided kernel: T, negative for the split exchange,
+// 100 * chunk + T for the chunked z pass
+int scode(const StridedKern& k)
+{
+    if (k.chunk > 0) return 100 * k.chunk + k.T;
+    return k.split ? -k.T : k.T;
+}
+
 // strided pass over the y axis (es = nx) or the z axis (es = nx*ny)
 int launch_strided(ose_ctx* c, int kind, const StridedKern& k, const float2* in, float2* out, int nx,
                    int ny, int nz, bool zaxis, int nshot, long shot_stride, const float2* tw,
@@ -605,6 +622,19 @@ int launch_strided(ose_ctx* c, int kind, const StridedKern& k, const float2* in,
     a.xcd = (double)grid.x * grid.y * grid.z < 16777216.0 && grid.x < 65536 && grid.y < 65536;
     a.mgx = (1ULL << 40) / grid.x + 1;
     a.mgy = (1ULL << 40) / grid.y + 1;
+    if (k.chunk > 0) {
+        // chunked z pass: a 1-D grid of chunks of k.chunk tiles; the kernel
+        // decodes tiles with the same exact division, so it needs its bounds
+        if (!zaxis || !m) return c->fail(OSE_EINVAL, "chunked strided kernels run the z symbol pass only");
+        if (!chunk_usable(k, nx, ny, nshot))
+            return c->fail(OSE_EUNSUPPORTED, "chunked z pass: %u x %u x %d tiles exceed its tile indexing", grid.x,
+                           grid.y, nshot);
+        a.tgx = (int)grid.x;
+        a.tgy = (int)grid.y;
+        a.ntiles = grid.x * grid.y * grid.z;
+        a.chunk = k.chunk;
+        grid = dim3((a.ntiles + k.chunk - 1) / k.chunk, 1, 1);
+    }
     if (m) {
         a.kx2 = m->d_kx2;
         a.ky2 = m->d_ky2;
@@ -998,18 +1028,21 @@ int autotune_model(ose_ctx* c, Model& m)
     const int reps = std::max(1, std::min(5, (int)(2e8 / ((double)m.ncell * nshot))));
     best = 1e30f;
     for (int v = 0; v < m.ey->ns && !rc; ++v) {
+        if (!m.ey->yf[v].fn) continue;   // a chunked z-pass slot
         rc = time_variant(c, [&] { return launch_strided(c, -1, m.ey->yf[v], m.X, m.S, nx, ny, nz, false, nshot,
                                                          m.ncell, m.d_twy, nullptr, 0.0); }, reps, &t);
         if (!rc && t < best) best = t, m.sel_yf = v;
     }
     best = 1e30f;
     for (int v = 0; v < m.ez->ns && !rc; ++v) {
+        if (!chunk_usable(m.ez->zs[v], nx, ny, nshot)) continue;
         rc = time_variant(c, [&] { return launch_strided(c, -1, m.ez->zs[v], m.S, m.X, nx, ny, nz, true, nshot,
                                                          m.ncell, m.d_twz, &m, 0.0); }, reps, &t);
         if (!rc && t < best) best = t, m.sel_z = v;
     }
     best = 1e30f;
     for (int v = 0; v < m.ey->ns && !rc; ++v) {
+        if (!m.ey->yi[v].fn) continue;
         rc = time_variant(c, [&] { return launch_strided(c, -1, m.ey->yi[v], m.X, m.S, nx, ny, nz, false, nshot,
                                                          m.ncell, m.d_twy, nullptr, 0.0); }, reps, &t);
         if (!rc && t < best) best = t, m.sel_yi = v;
@@ -1935,6 +1968,58 @@ int ose_timed_run(ose_ctx* c, int nsteps, double* ms)
     return collect(c);
 }
 
+int ose_pattern_probe(ose_ctx* c, int model_id, int axis, int reps, double* gbs)
+{
+    if (!c || !gbs || reps <= 0) return OSE_EINVAL;
+    if (model_id < 0 || model_id >= (int)c->models.size()) return c->fail(OSE_EINVAL, "ose_pattern_probe: bad model");
+    if (axis != 1 && axis != 2) return c->fail(OSE_EINVAL, "ose_pattern_probe: axis must be 1 (y) or 2 (z)");
+    int rc;
+    if ((rc = set_device(c)) || (rc = finalize(c)) || (rc = ose_sync(c))) return rc;
+    Model& m = *c->models[model_id];
+    const int nshot = (int)m.shots.size();
+    if (nshot == 0) return c->fail(OSE_ESTATE, "ose_pattern_probe: model has no shots");
+    const bool zaxis = axis == 2;
+    const FftEntry* e = zaxis ? m.ez : m.ey;
+    const StridedKern& k = zaxis ? m.ez->zs[m.sel_z] : m.ey->yf[m.sel_yf];
+    if (e->N1 <= 0 || e->N1 > 32 || e->N2 > 32 || (k.T != 8 && k.T != 16 && k.T != 32))
+        return c->fail(OSE_EUNSUPPORTED, "ose_pattern_probe: extent %d has no compiled split", e->N);
+    const int nx = m.d.nxb, ny = m.d.nyb, nz = m.d.nzb;
+    // the pass's own grid, tile order and occupancy (its LDS request); field
+    // X (the next step's x spectrum, read) to S (scratch between steps,
+    // written), so the state survives between steps
+    StridedArgs a{};
+    a.in = m.X;
+    a.out = m.S;
+    a.ncol = nx;
+    a.es = zaxis ? (long)nx * ny : (long)nx;
+    a.os = zaxis ? (long)nx : (long)nx * ny;
+    a.shot_stride = m.ncell;
+    const dim3 grid((nx + k.T - 1) / k.T, zaxis ? ny : nz, nshot);
+    a.xcd = (double)grid.x * grid.y * grid.z < 16777216.0 && grid.x < 65536 && grid.y < 65536;
+    a.mgx = (1ULL << 40) / grid.x + 1;
+    a.mgy = (1ULL << 40) / grid.y + 1;
+    int n1 = e->N1, n2 = e->N2;
+    const void* fn = k.T == 8 ? (const void*)&k_strided_copy<8>
+                   : k.T == 16 ? (const void*)&k_strided_copy<16> : (const void*)&k_strided_copy<32>;
+    const dim3 block(k.T * std::max(n1, n2));
+    if (k.lds > 65536) HIPCHK(c, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)k.lds));
+    void* args[] = {&a, &n1, &n2};
+    HIPCHK(c, hipLaunchKernel(fn, grid, block, args, k.lds, c->stream));   // warm-up
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; ++r) HIPCHK(c, hipLaunchKernel(fn, grid, block, args, k.lds, c->stream));
+    HIPCHK(c, hipEventRecord(e1, c->stream));
+    HIPCHK(c, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *gbs = 16.0 * (double)m.ncell * nshot * reps / (ms * 1e-3) / 1e9;
+    return OSE_OK;
+}
+
 int ose_copy_probe(ose_ctx* c, size_t bytes, int reps, double* gbs)
 {
     if (!c || !gbs || reps <= 0 || bytes < 4096) return OSE_EINVAL;
@@ -1989,7 +2074,16 @@ int ose_pass_probe(ose_ctx* c, int model_id, int which, int reps, double* ms)
     float2* F = m.W[1];
     switch (which) {
     case 0: body = [&] { return launch_strided(c, -1, m.ey->yf[m.sel_yf], F, F, nx, ny, nz, false, nshot, m.ncell, m.d_twy, nullptr, 0.0); }; break;
-    case 1: body = [&] { return launch_strided(c, -1, m.ez->yf[m.sel_z], F, F, nx, ny, nz, true, nshot, m.ncell, m.d_twz, nullptr, 0.0); }; break;
+    case 1: {
+        // forward-z alone in the z symbol's shape (a chunked z symbol has no
+        // forward-z kernel: the T = 16 split one stands in)
+        int v = m.sel_z;
+        for (int u = 0; !m.ez->yf[v].fn && u < m.ez->ns; ++u)
+            if (m.ez->yf[u].fn && m.ez->yf[u].T == m.ez->zs[m.sel_z].T) v = u;
+        if (!m.ez->yf[v].fn) v = 0;
+        body = [&, v] { return launch_strided(c, -1, m.ez->yf[v], F, F, nx, ny, nz, true, nshot, m.ncell, m.d_twz, nullptr, 0.0); };
+        break;
+    }
     case 2: body = [&] { return launch_strided(c, -1, m.ez->zs[m.sel_z], F, F, nx, ny, nz, true, nshot, m.ncell, m.d_twz, &m, 0.0); }; break;
     case 3: body = [&] { return launch_strided(c, -1, m.ey->yi[m.sel_yi], F, F, nx, ny, nz, false, nshot, m.ncell, m.d_twy, nullptr, 0.0); }; break;
     case 4: body = [&] { return launch_x(c, -1, m.ex->xk[m.sel_x], XM_MID, a, ny * nz, nshot, 0.0); }; break;
@@ -2026,10 +2120,9 @@ int ose_get_tuning(ose_ctx* c, int model_id, int* shape)
     if (!c || !shape) return OSE_EINVAL;
     if (model_id < 0 || model_id >= (int)c->models.size()) return c->fail(OSE_EINVAL, "ose_get_tuning: bad model");
     const Model& m = *c->models[model_id];
-    auto code = [](const StridedKern& k) { return k.split ? -k.T : k.T; };
-    shape[0] = code(m.ey->yf[m.sel_yf]);
-    shape[1] = code(m.ez->zs[m.sel_z]);
-    shape[2] = code(m.ey->yi[m.sel_yi]);
+    shape[0] = scode(m.ey->yf[m.sel_yf]);
+    shape[1] = scode(m.ez->zs[m.sel_z]);
+    shape[2] = scode(m.ey->yi[m.sel_yi]);
     shape[3] = xcode(m.ex->xk[m.sel_x]);
     return OSE_OK;
 }
@@ -2039,11 +2132,10 @@ int ose_set_tuning(ose_ctx* c, int model_id, const int* shape)
     if (!c || !shape) return OSE_EINVAL;
     if (model_id < 0 || model_id >= (int)c->models.size()) return c->fail(OSE_EINVAL, "ose_set_tuning: bad model");
     Model& m = *c->models[model_id];
-    auto code = [](const StridedKern& k) { return k.split ? -k.T : k.T; };
     auto find_strided = [&](const FftEntry* e, int which, int want) {
         for (int v = 0; v < e->ns; ++v) {
             const StridedKern& k = which == 0 ? e->yf[v] : which == 1 ? e->zs[v] : e->yi[v];
-            if (code(k) == want) return v;
+            if (k.fn && scode(k) == want) return v;
         }
         return -1;
     };
@@ -2115,9 +2207,10 @@ int ose_launch_shapes(ose_ctx* c, int model_id, int* strided, int* nstrided, int
     // strided shapes common to the y and z extents
     int n = 0;
     for (int v = 0; v < m.ey->ns; ++v) {
-        const int code = m.ey->yf[v].split ? -m.ey->yf[v].T : m.ey->yf[v].T;
+        if (!m.ey->yf[v].fn) continue;   // a chunked z-pass slot
+        const int code = scode(m.ey->yf[v]);
         bool inz = false;
-        for (int u = 0; u < m.ez->ns; ++u) inz |= (m.ez->zs[u].split ? -m.ez->zs[u].T : m.ez->zs[u].T) == code;
+        for (int u = 0; u < m.ez->ns; ++u) inz |= scode(m.ez->zs[u]) == code;
         if (inz) {
             if (strided && n < *nstrided) strided[n] = code;
             ++n;

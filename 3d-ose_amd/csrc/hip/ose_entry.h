@@ -7,9 +7,12 @@ namespace ose {
 struct GenPlan;
 
 // one compiled launch shape of a strided (y/z) pass: T x columns per
-// workgroup, split = 1 for the half-LDS (real, then imaginary) exchange
+// workgroup, split = 1 for the half-LDS (real, then imaginary) exchange,
+// chunk > 0 for the chunked z pass (k_zsym_chunk: that many consecutive
+// tiles per workgroup, each tile's successor prefetched; z symbol only, the
+// slot's forward and inverse y kernels are empty)
 struct StridedKern {
-    int T = 0, split = 0, threads = 0;
+    int T = 0, split = 0, threads = 0, chunk = 0;
     size_t lds = 0;
     const void* fn = nullptr;
     const GenPlan* gp = nullptr;      // runtime-length kernels: their plan

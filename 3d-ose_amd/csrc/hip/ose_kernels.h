@@ -75,6 +75,12 @@ struct StridedArgs {
     float invN;         // 1 / (nx ny nz)
     int xcd;            // 1: XCD-contiguous tile order (StridedTile)
     unsigned long long mgx, mgy;   // 2^40 / gridDim.x + 1 and 2^40 / gridDim.y + 1
+                                   // (chunked z pass: of tgx and tgy)
+    // chunked z pass only (k_zsym_chunk): tiles tgx (column blocks) x tgy
+    // (rows) x shots, `chunk` consecutive tiles per workgroup
+    int tgx, tgy;
+    unsigned ntiles;
+    int chunk;
     GenPlan g;          // runtime-length path only
 };
 
@@ -95,6 +101,7 @@ __device__ __forceinline__ unsigned div_magic(unsigned n, unsigned long long m)
 // (one set of pages, whole 256-byte segments) instead of blocks 8 apart.
 struct StridedTile {
     int bx, by, bz;
+    StridedTile() = default;
     __device__ __forceinline__ explicit StridedTile(const StridedArgs& a)
     {
         if (!a.xcd) {
@@ -270,6 +277,29 @@ void k_strided(StridedArgs a)
     }
 }
 
+// Access-pattern probe: the HBM reads and writes of a strided pass (pencil
+// rows N2 n1 + t of T columns, the tile order and addressing of k_strided
+// and k_zsym) with no arithmetic, for N1, N2 <= 32 given at run time.  It
+// measures what the pass's access pattern alone allows.
+template <int T>
+__global__ __launch_bounds__(T * 32) void k_strided_copy(StridedArgs a, int N1, int N2)
+{
+    const int tid = threadIdx.x;
+    const int col = tid % T;
+    const int t = tid / T;
+    const StridedTile g(a);
+    const StridedAddr ad(a, g, T, col, t);
+    if (t >= N2) return;
+    float2 v[32];
+#pragma unroll
+    for (int n1 = 0; n1 < 32; ++n1)
+        if (n1 < N1) v[n1] = ad.load(a.in, N2 * n1);
+    if (!ad.ok) return;
+#pragma unroll
+    for (int n1 = 0; n1 < 32; ++n1)
+        if (n1 < N1) ad.store(a.out, N2 * n1, v[n1]);
+}
+
 // z pass: forward-z, multiply by |k|/N (folded wavenumbers, scale_kxkz_3D
 // ose_3d_kernel.cu:101-126 and the 1/N of scal_vr_3d :355), inverse-z.
 // In place; the tile row (StridedTile::by) is the (spectral) y index.  Each LDS location of the
@@ -326,6 +356,129 @@ void k_zsym(StridedArgs a)
 #pragma unroll
             for (int n1 = 0; n1 < N1; ++n1) ad.store(a.out, N2 * n1, v[n1]);
         }
+    }
+}
+
+// Chunked z pass: the arithmetic of k_zsym (whole-complex exchange) on
+// a.chunk consecutive tiles per workgroup, chunks in XCD-contiguous order.
+// Each tile's successor is loaded into registers before the tile is
+// transformed, so the next tile's HBM reads overlap this tile's work: the z
+// pencils' reads cross a 6 MB page stride per element and take the longest
+// address translation of the step, which one tile per workgroup (two
+// workgroups per CU) leaves exposed.  The twiddle rows and the squared z
+// wavenumbers are staged in LDS once per workgroup: read from global memory
+// inside the loop they would queue behind the prefetched reads (the vector
+// memory counter completes in order) and stall every tile on them.  One
+// workgroup per CU (its registers hold a tile and a half).
+template <int N1, int N2, int T>
+struct ZChunkCfg {
+    static constexpr int N = N1 * N2;
+    static constexpr int NT = Max<N1, N2>::value;
+    static constexpr int THREADS = T * NT;
+    static constexpr int LDT = T + 1;
+    static constexpr size_t XCH = (size_t)N * LDT;   // exchange, float2
+    static constexpr size_t LDS_BYTES = sizeof(float2) * (XCH + 2 * (size_t)N) + sizeof(float) * (size_t)N;
+};
+
+// tile L of the chunked z pass: column block fastest, then row, then shot
+__device__ __forceinline__ StridedTile zc_tile(const StridedArgs& a, unsigned L)
+{
+    const unsigned row = div_magic(L, a.mgx);
+    const unsigned shot = div_magic(row, a.mgy);
+    StridedTile g;
+    g.bx = (int)(L - row * (unsigned)a.tgx);
+    g.by = (int)(row - shot * (unsigned)a.tgy);
+    g.bz = (int)shot;
+    return g;
+}
+
+// issue tile L's reads: the column's kx^2 and the row's ky^2, then the
+// pencil values.  With step = 0 every read hits the tile's first row (the
+// prefetch after the chunk's last tile: cache hits, results unused), so the
+// reads are issued unconditionally and the wait before a tile's work stays
+// exact (a conditional prefetch made the compiler wait for every read).
+template <int N1, int N2, int T>
+__device__ __forceinline__ void zc_fetch(const StridedArgs& a, unsigned L, int step, int col, int t,
+                                         float2 (&vn)[N1], float& kxn, float& kyn)
+{
+    const StridedTile g = zc_tile(a, L);
+    const StridedAddr ad(a, g, T, col, t);
+    kxn = a.kx2[g.bx * T + ad.xc];
+    kyn = a.ky2[g.by];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) vn[n1] = ad.load(a.in, step * N2 * n1);
+}
+
+template <int N1, int N2, int T>
+__global__ __launch_bounds__((ZChunkCfg<N1, N2, T>::THREADS), 1)
+void k_zsym_chunk(StridedArgs a)
+{
+    using C = ZChunkCfg<N1, N2, T>;
+    static_assert(N1 <= N2, "every thread of the chunked z pass loads a pencil slice");
+    extern __shared__ float2 lds[];
+    float2* ltw = lds + C::XCH;     // w^(t k1) at t N1 + k1; w^(t n2) at N + t N2 + n2
+    float* lkz = reinterpret_cast<float*>(ltw + 2 * C::N);
+    const int tid = threadIdx.x;
+    const int col = tid % T;
+    const int t = tid / T;
+    const unsigned n = gridDim.x, id = blockIdx.x;
+    const unsigned x = id % 8, q = n / 8, r = n % 8;
+    const unsigned first = (x * q + min(x, r) + id / 8) * (unsigned)a.chunk;
+    const unsigned last = min(first + (unsigned)a.chunk, a.ntiles);
+    if (first >= last) return;
+    for (int i = tid; i < 2 * C::N; i += C::THREADS) ltw[i] = a.tw[C::N + i];
+    for (int i = tid; i < C::N; i += C::THREADS) lkz[i] = a.kz2[i];
+    __syncthreads();
+    float2 v[C::NT];
+    float2 vn[N1];
+    float kxn, kyn;
+    zc_fetch<N1, N2, T>(a, first, 1, col, t, vn, kxn, kyn);
+    for (unsigned L = first; L < last; ++L) {
+        const StridedTile g = zc_tile(a, L);
+        const StridedAddr ad(a, g, T, col, t);
+        const float kxy = kxn + kyn;
+#pragma unroll
+        for (int i = 0; i < N1; ++i) v[i] = vn[i];
+        const bool more = L + 1 < last;
+        zc_fetch<N1, N2, T>(a, more ? L + 1 : L, more ? 1 : 0, col, t, vn, kxn, kyn);
+        __builtin_amdgcn_sched_barrier(0);
+        {
+            rdft<N1, -1>(v);
+            const float2* twt = ltw + t * N1;
+#pragma unroll
+            for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], twt[k1]);
+        }
+        col_exchange<T, 0, N1, N2>(
+            v, true, t < N1, [&](int k1) { return t * N1 + k1; }, [&](int n2) { return n2 * N1 + t; }, lds, col);
+        if (t < N1) {
+            rdft<N2, -1>(v);
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) {
+                const float s = __builtin_amdgcn_sqrtf(kxy + lkz[t + N1 * k2]) * a.invN;
+                v[k2].x *= s;
+                v[k2].y *= s;
+            }
+            rdft<N2, 1>(v);
+            const float2* twi = ltw + C::N + t * N2;
+#pragma unroll
+            for (int n2 = 1; n2 < N2; ++n2) v[n2] = cmulc(v[n2], twi[n2]);
+        }
+        col_exchange<T, 0, N2, N1>(
+            v, t < N1, true, [&](int n2) { return n2 * N1 + t; }, [&](int m1) { return t * N1 + m1; }, lds, col);
+        {
+            rdft<N1, 1>(v);
+            // Unconditional stores: a thread past the last column addresses
+            // the last column (StridedAddr) and has transformed that same
+            // pencil with the same instructions, so it writes the owner's
+            // values bit for bit.  Conditional stores would make the compiler
+            // wait for this tile's writes before the next tile's prefetched
+            // values could be used (it cannot count writes that may not
+            // have been issued).
+#pragma unroll
+            for (int n1 = 0; n1 < N1; ++n1) ad.store(a.out, N2 * n1, v[n1]);
+        }
+        // the next tile's first exchange overwrites what this tile's second read
+        __syncthreads();
     }
 }
 

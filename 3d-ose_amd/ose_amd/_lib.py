@@ -91,6 +91,7 @@ def hip():
         _proto(L, "ose_pass_stats", [vp, c_int, dp, P(c_long), dp])
         _proto(L, "ose_reset_stats", [vp])
         _proto(L, "ose_copy_probe", [vp, ctypes.c_size_t, c_int, dp])
+        _proto(L, "ose_pattern_probe", [vp, c_int, c_int, c_int, dp])
         _proto(L, "ose_pass_probe", [vp, c_int, c_int, c_int, dp])
         _proto(L, "ose_set_autotune", [vp, c_int])
         _proto(L, "ose_get_tuning", [vp, c_int, ip])
@@ -144,7 +145,7 @@ def hip_symbols():
         "ose_fft3d", "ose_set_options", "ose_pass_stats", "ose_reset_stats",
         "ose_supported_extent", "ose_extent_is_supported", "ose_extent_kind", "ose_set_fft_path", "ose_set_autotune", "ose_get_tuning",
         "ose_set_tuning", "ose_get_tuning_modes", "ose_set_tuning_modes", "ose_launch_shapes",
-        "ose_copy_probe", "ose_pass_probe", "ose_boundary_net_size", "ose_set_boundary_net",
+        "ose_copy_probe", "ose_pattern_probe", "ose_pass_probe", "ose_boundary_net_size", "ose_set_boundary_net",
         "ose_boundary_status", "ose_trip_boundary_net", "ose_bnet_create", "ose_bnet_destroy", "ose_bnet_error",
         "ose_bnet_set_params", "ose_bnet_get_params", "ose_bnet_forward", "ose_bnet_gradient",
         "ose_bnet_add_sample", "ose_bnet_train", "ose_accumulation_plan", "ose_set_streams",

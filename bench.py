@@ -348,6 +348,25 @@ def snapshot_leg(params, steps, every, warmup=2):
     return {"params": params, "models": info["models"], "shots": shots, "steps": steps, "runs": out}
 
 
+def pattern_bounds(sim, passes, reps=3):
+    """What the strided passes' access patterns alone allow: the reads and
+    writes of forward-y and of the z symbol pass of the first model, in their
+    launch shapes, without the transforms (ose_pattern_probe), beside each
+    pass's own rate in the step."""
+    import ctypes
+    from ose_amd import _lib
+    L = _lib.hip()
+    out = {}
+    for axis, name in ((1, "fwd_y"), (2, "z_symbol")):
+        g = ctypes.c_double()
+        if L.ose_pattern_probe(sim.ctx, 0, axis, reps, ctypes.byref(g)):
+            return None
+        own = passes.get(name, {}).get("GBs", 0.0)
+        out[name] = {"access_only_GBs": round(g.value, 1), "pass_GBs": round(own, 1),
+                     "frac_of_access_bound": round(own / g.value, 3) if g.value else None}
+    return out
+
+
 def shape_sweep(params, reps=5):
     """Every compiled strided launch shape of the workload's first model,
     pinned in turn: mean HIP-event duration of one launch of forward-y,
@@ -381,6 +400,19 @@ def shape_sweep(params, reps=5):
             if L.ose_pass_probe(ctx, 0, which, reps, ctypes.byref(ms)):
                 raise RuntimeError(L.ose_last_error(ctx).decode())
             row[name] = {"ms": round(ms.value, 3), "TBs": round(16.0 * cells / (ms.value * 1e-3) / 1e12, 3)}
+        rows.append(row)
+        print(json.dumps(row), flush=True)
+    # z-only shapes: the chunked z symbol (100 * tiles per workgroup + T),
+    # the y passes in their autotuned shapes
+    for code in (216, 316, 416, 616):
+        t = (ctypes.c_int * 4)(base[0], code, base[2], base[3])
+        if L.ose_set_tuning(ctx, 0, t):
+            continue
+        ms = ctypes.c_double()
+        if L.ose_pass_probe(ctx, 0, 2, reps, ctypes.byref(ms)):
+            raise RuntimeError(L.ose_last_error(ctx).decode())
+        row = {"code": code, "z_symbol": {"ms": round(ms.value, 3),
+                                          "TBs": round(16.0 * cells / (ms.value * 1e-3) / 1e12, 3)}}
         rows.append(row)
         print(json.dumps(row), flush=True)
     sim.close()
@@ -469,6 +501,9 @@ def main():
                 "frac": pd["GBs"] / HBM_PEAK_GBS, "traffic": traffic_from_profile(args.config, dom), "kernel": dom,
                 "bytes_per_launch": pd["bytes"], "bytes_rule": PASS_BYTES_DOC[dom],
                 "passes": {k: round(v["GBs"], 1) for k, v in passes.items()}}
+    pattern = pattern_bounds(sim, passes)
+    if pattern:
+        roofline["access_pattern"] = pattern
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(sim, info, args.cpu_seconds)

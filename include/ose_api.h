@@ -247,6 +247,16 @@ int ose_launch_shapes(ose_ctx* ctx, int model_id, int* strided, int* nstrided, i
  * repeated reps times; gbs = (read + write bytes) / time.  No counterpart in
  * the reference; it calibrates the per-pass GB/s and the HBM counters.    */
 int ose_copy_probe(ose_ctx* ctx, size_t bytes, int reps, double* gbs);
+/* Access-pattern bound of a strided pass: the HBM reads and writes of model
+ * model_id's forward-y (axis 1) or z-symbol (axis 2) pass over all its
+ * shots, from one of the model's two spectral work fields to the other
+ * (the state is intact between steps), in the pass's current launch shape
+ * (same tile order, addressing and workgroups per CU) but without the
+ * transforms, repeated reps times; gbs = (read + write bytes) / time.  The
+ * pass's own rate over this is the share of its access pattern's bound it
+ * reaches.  Compiled extents only (OSE_EUNSUPPORTED on the runtime-length
+ * path).  No counterpart in the reference.                               */
+int ose_pattern_probe(ose_ctx* ctx, int model_id, int axis, int reps, double* gbs);
 
 /* Mean duration (ms) of one launch of a single pass over all shots of a
  * model, on scratch buffers: which = 0 forward-y, 1 plain forward-z,

@@ -122,6 +122,80 @@ def test_every_launch_shape_gives_the_same_step(padded):
         assert rel(ref, P + 1j * Q) < 1e-5
 
 
+@pytest.mark.parametrize("padded,nshot", [((48, 40, 336), 2), ((40, 22, 336), 3)])
+def test_chunked_z_pass_equals_one_tile_per_workgroup(padded, nshot):
+    """The chunked z-symbol pass (several consecutive tiles per workgroup,
+    each tile's successor prefetched; codes 100 * tiles + 16) computes what
+    the one-tile-per-workgroup pass computes, on every shot: chunks that
+    cross rows and shots, a last chunk shorter than the rest (3 x 22 x 3 =
+    198 tiles in chunks of 4) and a last column block with threads past the
+    grid (40 = 2 x 16 + 8 columns, whose duplicate stores write the owner's
+    values); the one-tile result agrees with the FP64 oracle."""
+    import ose_amd
+    nxb, nyb, nzb = padded
+    print(f"chunked z pass {padded} x {nshot}", flush=True)
+    case = Case((nxb - 12, nyb - 12, nzb - 12), (6, 6, 6, 6, 6, 6), layered(nxb - 12, nyb - 12, nzb - 12, seed=17))
+    us = [_smooth_field(case.shape, 30 + k).astype(np.complex64) for k in range(nshot)]
+    results = {}
+    with ose_amd.Context(0) as c:
+        m = case.add_to(c)
+        sids = [c.add_shot(m, (case.bl, case.bf, case.bu), np.zeros(0), np.zeros(0)) for _ in range(nshot)]
+        c.run(0)
+        base = c.tuning(m)
+        for zcode in (-16, 216, 316, 416, 616):
+            c.set_tuning(m, base["T_fwd_y"], zcode, base["T_inv_y"], base["P_x"])
+            assert c.tuning(m)["T_z"] == zcode
+            for sid, u in zip(sids, us):
+                c.set_wavefield(sid, u.real, u.imag)
+            c.step = 0
+            c.run(1)
+            results[zcode] = [complex_field(c.wavefield(sid)) for sid in sids]
+    for zcode, fields in results.items():
+        for k in range(nshot):
+            assert rel(fields[k], results[-16][k]) < 1e-6, (zcode, k)
+    for k in range(nshot):
+        P, Q = case.oracle().step(us[k].real.astype(np.float64), us[k].imag.astype(np.float64))
+        assert rel(results[-16][k], P + 1j * Q) < 1e-5, k
+
+
+def complex_field(reim):
+    re, im = reim
+    return re + 1j * im
+
+
+@pytest.mark.parametrize("nshot", [1, 4])
+def test_pattern_probe_leaves_the_state_intact(nshot):
+    """ose_pattern_probe (the strided passes' reads and writes without the
+    transforms) reports a rate for both axes, and a run that probes between
+    its steps ends where the plain run ends, bit for bit: in place (one
+    336^3 shot) and with the second spectral field (four shots)."""
+    import ctypes
+    import ose_amd
+    print(f"pattern probe between steps, {nshot} shot(s) of 336^3", flush=True)
+    nt = 4
+    case = Case((324, 324, 324), (6, 6, 6, 6, 6, 6), layered(324, 324, 324, seed=41))
+    shots = _shots(case, [(162, 162, 2, 30.0, 2), (40, 200, 10, 25.0, 3), (300, 20, 30, 35.0, 2),
+                          (100, 100, 100, 28.0, 1)], nt)[:nshot]
+    out = {}
+    for probe in (False, True):
+        with ose_amd.Context(0) as c:
+            c.set_autotune(False)
+            m = case.add_to(c)
+            ids = [c.add_shot(m, (case.bl + sx, case.bf + sy, case.bu + sz), s, h) for (sx, sy, sz), s, h in shots]
+            for _ in range(nt):
+                c.run(1)
+                if probe:
+                    for axis in (1, 2):
+                        g = ctypes.c_double()
+                        assert c._L.ose_pattern_probe(c._h, m, axis, 1, ctypes.byref(g)) == 0
+                        assert g.value > 0
+            out[probe] = [(c.traces(i), c.wavefield(i)) for i in ids]
+    for (t0, (r0, i0)), (t1, (r1, i1)) in zip(out[False], out[True]):
+        assert np.abs(r0).max() > 0
+        assert np.array_equal(t0, t1)
+        assert np.array_equal(r0, r1) and np.array_equal(i0, i1)
+
+
 @pytest.mark.parametrize("use_graph", [True, False])
 def test_two_stream_lanes_equal_one_stream(use_graph):
     """The step's two concurrent lanes (shots dealt in halves per model,
