@@ -7,7 +7,8 @@ sizes.  It also writes dft_gen.h (the register DFTs for every N1, N2) and
 ose_sizes.inc (the registry list).
 
 SIZES are the extents the configurations use (plus 152 = 8*19, the
-reference's own 101+25+26 grid), compiled in every launch shape the
+reference's own 101+25+26 grid, and 832, where the 768^3 workload's 24-cell
+taper lands: 816 has the prime factor 17), compiled in every launch shape the
 autotuner chooses from; LEAN are the other even extents up to 1024 with
 small prime factors, one shape each.  Every other extent up to 4096 runs
 on the runtime-length path (ose_generic.h); the host's extent=smooth rounds
@@ -31,7 +32,7 @@ SIZES = {
     336: (16, 21), 378: (18, 21), 384: (16, 24), 400: (20, 20), 448: (16, 28),
     480: (20, 24), 512: (16, 32), 576: (24, 24), 600: (20, 30), 640: (20, 32),
     672: (21, 32), 720: (24, 30), 768: (24, 32), 784: (28, 28), 800: (25, 32),
-    864: (27, 32), 960: (30, 32), 1024: (32, 32),
+    832: (26, 32), 864: (27, 32), 960: (30, 32), 1024: (32, 32),
 }
 
 # radices of the runtime-length path's register butterflies (ose_generic.h)

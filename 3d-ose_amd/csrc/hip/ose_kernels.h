@@ -6,15 +6,19 @@
 // Im = q-lane, ose_3d_lib.cu:639-874) and runs the three-term recursion
 //     W_{j+1} = W_{j-1} - 2i L W_j,   A = g (A + c_{j+1} W_{j+1}).
 // It is four HBM passes over complex64 fields laid out [shot][z][y][x]:
-//   1. k_strided<SIGN=-1>  forward-y       X_j  -> S
-//   2. k_zsym              forward-z, * |k|/N, inverse-z   (in place on S)
-//   3. k_strided<SIGN=+1>  inverse-y       (in place on S)
+//   1. k_strided<SIGN=-1>  forward-y       X -> S
+//   2. k_zsym              forward-z, * |k|/N, inverse-z   S -> X
+//                          (or k_zsym_chunk: several tiles per workgroup)
+//   3. k_strided<SIGN=+1>  inverse-y       X -> S
 //   4. k_xpass<MODE>       inverse-x of S, recursion on physical W, taper,
 //                          accumulator (on about half of the orders, two
 //                          terms at a time), receivers, forward-x of W_{j+1}
-// Passes 1-3 run in place on X_j = FFT_x W_j; pass 4 reads W_{j-1} in
-// physical form and overwrites it with W_{j+1} (two physical W buffers
-// alternate), so each order needs one inverse and one forward x transform.
+//                          into X
+// X starts as X_j = FFT_x W_j; passes 1-3 alternate between the two spectral
+// fields X and S (or run in place where a batch is small enough for S to be
+// X); pass 4 reads W_{j-1} in physical form and overwrites it with W_{j+1}
+// (two physical W buffers alternate), so each order needs one inverse and
+// one forward x transform.
 //
 // Every 1D transform of length N = N1*N2 is done in registers and LDS:
 // step A: N2 threads each run an N1-point register DFT over x[N2*n1 + t],
@@ -302,7 +306,8 @@ __global__ __launch_bounds__(T * 32) void k_strided_copy(StridedArgs a, int N1, 
 
 // z pass: forward-z, multiply by |k|/N (folded wavenumbers, scale_kxkz_3D
 // ose_3d_kernel.cu:101-126 and the 1/N of scal_vr_3d :355), inverse-z.
-// In place; the tile row (StridedTile::by) is the (spectral) y index.  Each LDS location of the
+// In to out (the same field where a batch runs in place); the tile row
+// (StridedTile::by) is the (spectral) y index.  Each LDS location of the
 // second exchange is written by the thread that read it in the first, so
 // the exchanges need no barrier between them.
 // The split form asks for at least 4 waves per SIMD (at most 128 VGPRs):
@@ -510,7 +515,7 @@ enum XMode : int {
 // The recursion runs on physical W: pass 4 of order j reads W_{j-1} and
 // writes W_{j+1} over it (Win == Wout except where noted), so only the
 // spectrum S needs an inverse x transform; the new W is forward transformed
-// into X for the next order's passes 1-3, which run in place on X.
+// into X for the next order's passes 1-3.
 struct XArgs {
     const float2* S;        // spectrum after passes 1-3 (natural order along x)
     const float2* Win;      // physical W_{j-1} (W_0 in ORDER0)

@@ -35,9 +35,9 @@ CONFIGS = {
     "ref": ("preset=ref", "reference as shipped: 101^3 two-layer, 25-cell pads"),
 }
 PASS_BYTES_DOC = {
-    "fwd_y": "16 B/cell (read X_j, write it in place)",
-    "z_symbol": "16 B/cell (read, write in place)",
-    "inv_y": "16 B/cell (read, write in place: S)",
+    "fwd_y": "16 B/cell (read X, write S; in place on X for batches under 1 GiB per field)",
+    "z_symbol": "16 B/cell (read S, write X; in place on X for batches under 1 GiB per field)",
+    "inv_y": "16 B/cell (read X, write S; in place on X for batches under 1 GiB per field)",
     "x_epilogue": "52 B/cell (read S, W_{j-1}, v/R, A; write X_{j+1}, W_{j+1}, A)",
     "x_acc_first": "44 B/cell (read S, W_{j-1}, v/R; write X_{j+1}, W_{j+1}, A: the order that starts A)",
     "x_recur": "36 B/cell (read S, W_{j-1}, v/R; write X_{j+1}, W_{j+1}; accumulator not touched)",
@@ -146,35 +146,47 @@ def set_streams(sim, n):
         raise RuntimeError(f"ose_set_streams({n}) failed: {rc}")
 
 
-LEARNED_PROFILE = os.path.join(ROOT, "profiles", "learned_pad_unreflected_64cube_400steps.txt")
+LEARNED_PROFILE = os.path.join(ROOT, "profiles", "learned_pad_unreflected_256cube_600steps.txt")
+# learned pads quoted at equal accuracy: (cells, weights file)
+LEARNED_PADS = ((8, "bnet_thin8.npy"), (16, "bnet_thin16.npy"))
 
 
 def learned_accuracy():
     """Medians of the committed comparison against the unreflected
-    reference: seismogram rel-L2 per boundary, and which plain taper width
-    the 8-cell learned pad matches (the widest plain taper whose median
-    error is not below the learned pad's)."""
+    reference (256^3 interior, 20 held-out models): seismogram rel-L2 per
+    boundary, and for each learned pad the plain taper width it matches
+    (the widest plain taper whose median error is not below the learned
+    pad's)."""
     import numpy as np
     rows = [json.loads(line) for line in open(LEARNED_PROFILE) if line.startswith("{")]
     keys = [k for k in rows[0] if k.startswith(("plain", "learned")) and not k.endswith("_status")]
     med = {k: float(np.median([r[k] for r in rows])) for k in keys}
-    l8 = med["learned8_bnet_thin8"]
     widths = sorted(int(k[5:]) for k in med if k.startswith("plain"))
-    match = max((w for w in widths if med[f"plain{w}"] >= l8), default=None)
+    match = {}
+    for cells, wfile in LEARNED_PADS:
+        key = f"learned{cells}_{os.path.splitext(wfile)[0]}"
+        match[cells] = (key, max((w for w in widths if med[f"plain{w}"] >= med[key]), default=None))
     return med, match, len(rows)
 
 
-def learned_leg(params, steps, warmup, plain_value, weights, streams):
-    """The same workload with an 8-cell pad: plain thin taper, then the
-    learned pad (network in the step, replacing the taper); and the plain
-    taper of the width whose accuracy the learned pad matches, for the speed
-    ratio at equal accuracy."""
+def learned_leg(params, steps, warmup, plain_value, streams):
+    """The same workload with thin pads: the plain 8-cell taper, the
+    learned 8- and 16-cell pads (network in the step, replacing the taper)
+    and, for each, the plain taper of the width whose accuracy it matches,
+    for the speed ratio at equal accuracy.  Padded extents follow the
+    workload's own extent rule (the smallest autotuned extent that holds
+    interior and pads)."""
     from ose_amd.host import Simulation
+    from ose_amd.boundary import WEIGHTS
+    wdir = os.path.dirname(WEIGHTS)
     med, match, nseeds = learned_accuracy()
     res = {}
-    legs = [("thin_taper", "boundary=8", ""), ("learned", "boundary=8", f"bnet={weights}")]
-    if match and match != 8:
-        legs.append((f"plain{match}", f"boundary={match}", ""))
+    legs = [("thin_taper", "boundary=8", "")]
+    for cells, wfile in LEARNED_PADS:
+        legs.append((f"learned{cells}", f"boundary={cells}", f"bnet={os.path.join(wdir, wfile)}"))
+        m = match[cells][1]
+        if m and m != 8 and all(name != f"plain{m}" for name, _, _ in legs):
+            legs.append((f"plain{m}", f"boundary={m}", ""))
     for name, pad, extra in legs:
         sim = Simulation(f"{params} {pad} nt={warmup + steps + 1} {extra}")
         sim.setup()
@@ -186,8 +198,16 @@ def learned_leg(params, steps, warmup, plain_value, weights, streams):
                      "padded": info["models"][0]["padded"]}
         print(f"[bench] {name}: {ms / steps:.1f} ms/step", file=sys.stderr, flush=True)
         sim.close()
-    lr = res["learned"]
-    eq = res.get(f"plain{match}")
+    lr = res["learned8"]
+    eq = []
+    for cells, wfile in LEARNED_PADS:
+        key, m = match[cells]
+        lc, pc = res[f"learned{cells}"], res.get(f"plain{m}")
+        eq.append({"learned_cells": cells, "weights": wfile, "learned_value": lc["value"],
+                   "learned_padded": lc["padded"], "median_rel_l2": med[key], "matches_plain_taper_cells": m,
+                   "plain_median_rel_l2": med.get(f"plain{m}"), "plain_value": pc["value"] if pc else None,
+                   "plain_padded": pc["padded"] if pc else None,
+                   "speed_ratio": lc["value"] / pc["value"] if pc else None})
     return {
         "value": lr["value"],
         "unit": "Gcell*steps/s",
@@ -196,20 +216,18 @@ def learned_leg(params, steps, warmup, plain_value, weights, streams):
         "vs_plain": lr["value"] / plain_value,
         "thin_taper_value": res["thin_taper"]["value"],
         "network_ms_per_step": lr["ms_per_step"] - res["thin_taper"]["ms_per_step"],
-        "weights": os.path.relpath(weights, ROOT),
-        "equal_accuracy": {
-            "matches_plain_taper_cells": match,
-            "plain_value": eq["value"] if eq else None,
-            "plain_padded": eq["padded"] if eq else None,
-            "speed_ratio": lr["value"] / eq["value"] if eq else None,
-            "reaches_plain40": med["learned8_bnet_thin8"] <= med["plain40"],
-        },
-        "accuracy": (f"seismogram rel-L2 against an unreflected reference, median over {nseeds} held-out 64^3 models "
-                     f"x 400 steps: learned 8-cell {med['learned8_bnet_thin8']:.3f}, plain taper "
+        "weights": os.path.relpath(WEIGHTS, ROOT),
+        "equal_accuracy": eq,
+        "reaches_plain40": any(med[match[c][0]] <= med["plain40"] for c, _ in LEARNED_PADS),
+        "accuracy": (f"seismogram rel-L2 against an unreflected reference, median over {nseeds} held-out 256^3 models "
+                     f"x 600 steps: "
+                     + ", ".join(f"learned {c}-cell {med[match[c][0]]:.3f}" for c, _ in LEARNED_PADS)
+                     + ", plain taper "
                      + ", ".join(f"{k[5:]}-cell {v:.3f}" for k, v in sorted(med.items(), key=lambda kv: -kv[1])
                                  if k.startswith("plain"))
-                     + f"; the learned pad matches the {match}-cell taper, no learned pad up to 16 cells reaches "
-                       f"the 40-cell one (the 1e-3 bar is not met by any boundary); "
+                     + "; " + "; ".join(f"learned {c}-cell matches the {match[c][1]}-cell taper"
+                                        for c, _ in LEARNED_PADS)
+                     + "; none reaches the 40-cell taper (the 1e-3 bar is not met by any boundary); "
                      + os.path.relpath(LEARNED_PROFILE, ROOT)),
     }
 
@@ -435,7 +453,7 @@ def main():
     ap.add_argument("--extra", default="", help="extra key=value parameters")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-learned", action="store_true", help="skip the 8-cell learned-pad leg")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-pad legs")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the accuracy leg")
     ap.add_argument("--streams", type=int, default=2, choices=(1, 2),
                     help="2: the step's shots run as two concurrent lanes (default); 1: one stream")
@@ -516,8 +534,7 @@ def main():
         print(f"[bench] accuracy: {json.dumps(accuracy)}", file=sys.stderr, flush=True)
     learned = None
     if not args.no_learned and args.config in ("c4", "c1"):
-        from ose_amd.boundary import WEIGHTS
-        learned = learned_leg(params + " " + args.extra, args.steps, args.warmup, value, WEIGHTS, args.streams)
+        learned = learned_leg(params + " " + args.extra, args.steps, args.warmup, value, args.streams)
     out = {
         "metric": METRIC,
         "value": value,
