@@ -113,6 +113,25 @@ def test_learned_pad_against_unreflected_reference():
         assert r["learned8"] < r["plain8"] and r["learned8"] < r["plain16"]
 
 
+def test_learned_pads_match_wider_tapers_at_256cube():
+    """Held-out seed 2015 at the 256^3 configuration's interior, 600 steps,
+    against the unreflected reference: the 8-cell learned pad is more
+    accurate than the 16-cell taper and the 16-cell learned pad
+    (bnet_thin16.npy) than the 24-cell taper, the pairs the benchmark
+    compares at equal accuracy (18 of 20 held-out models each,
+    profiles/learned_pad_unreflected_256cube_600steps.txt)."""
+    from ose_amd import boundary
+    p8 = np.load(boundary.WEIGHTS)
+    p16 = np.load(os.path.join(os.path.dirname(boundary.WEIGHTS), "bnet_thin16.npy"))
+    print("learned pads vs tapers at 256^3, seed 2015", flush=True)
+    r = boundary.compare_pads([2015], [("plain16", 16, None), ("plain24", 24, None), ("learned8", 8, p8),
+                                       ("learned16", 16, p16)], n=256, nt=600)[0]
+    print(r, flush=True)
+    assert not r["learned8_status"]["tripped"] and not r["learned16_status"]["tripped"]
+    assert r["learned8"] < r["plain16"]
+    assert r["learned16"] < r["plain24"]
+
+
 def test_unreflected_reference_matches_fp64_oracle():
     """The unreflected reference of the learned-pad comparison (seed 2018,
     taper-free 200^3 domain) over its first steps against the FP64 oracle
