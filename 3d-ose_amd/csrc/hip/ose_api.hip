@@ -1986,7 +1986,14 @@ int ose_pattern_probe(ose_ctx* c, int model_id, int axis, int reps, double* gbs)
     if (nshot == 0) return c->fail(OSE_ESTATE, "ose_pattern_probe: model has no shots");
     const bool zaxis = axis == 2;
     const FftEntry* e = zaxis ? m.ez : m.ey;
-    const StridedKern& k = zaxis ? m.ez->zs[m.sel_z] : m.ey->yf[m.sel_yf];
+    int sel = zaxis ? m.sel_z : m.sel_yf;
+    if (zaxis && m.ez->zs[sel].chunk > 0) {
+        // a chunked z pass: its one-tile counterpart of the same width (split exchange)
+        for (int u = 0; u < m.ez->ns; ++u)
+            if (m.ez->zs[u].chunk == 0 && m.ez->zs[u].T == m.ez->zs[sel].T && m.ez->zs[u].split) sel = u;
+        if (m.ez->zs[sel].chunk > 0) sel = 0;
+    }
+    const StridedKern& k = zaxis ? m.ez->zs[sel] : m.ey->yf[sel];
     if (e->N1 <= 0 || e->N1 > 32 || e->N2 > 32 || (k.T != 8 && k.T != 16 && k.T != 32))
         return c->fail(OSE_EUNSUPPORTED, "ose_pattern_probe: extent %d has no compiled split", e->N);
     const int nx = m.d.nxb, ny = m.d.nyb, nz = m.d.nzb;

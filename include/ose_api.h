@@ -251,7 +251,8 @@ int ose_copy_probe(ose_ctx* ctx, size_t bytes, int reps, double* gbs);
  * model_id's forward-y (axis 1) or z-symbol (axis 2) pass over all its
  * shots, from one of the model's two spectral work fields to the other
  * (the state is intact between steps), in the pass's current launch shape
- * (same tile order, addressing and workgroups per CU) but without the
+ * (same tile order, addressing and workgroups per CU; for a chunked z pass
+ * its one-tile counterpart of the same width) but without the
  * transforms, repeated reps times; gbs = (read + write bytes) / time.  The
  * pass's own rate over this is the share of its access pattern's bound it
  * reaches.  Compiled extents only (OSE_EUNSUPPORTED on the runtime-length
