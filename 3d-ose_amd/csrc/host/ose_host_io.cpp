@@ -156,7 +156,8 @@ int oseh_read_su(const char* path, float* traces, size_t cap, int* nrec, int* nt
     double d = 0.0;
     size_t used = 0;
     int rc = OSE_OK;
-    while (std::fread(h, 1, sizeof h, f) == sizeof h) {
+    size_t got;
+    while ((got = std::fread(h, 1, sizeof h, f)) == sizeof h) {
         const int tns = get<uint16_t>(h, 114);
         if (ns < 0) {
             ns = tns;
@@ -178,6 +179,8 @@ int oseh_read_su(const char* path, float* traces, size_t cap, int* nrec, int* nt
         used += (size_t)ns;
         ++n;
     }
+    // a file that ends inside a trace header is cut short, not complete
+    if (rc == OSE_OK && got != sizeof h && got != 0) rc = OSE_EINVAL;
     std::fclose(f);
     if (nrec) *nrec = n;
     if (nt) *nt = ns < 0 ? 0 : ns;
@@ -263,8 +266,8 @@ int oseh_read_segy(const char* path, float* traces, size_t cap, int* nrec, int* 
     unsigned char h[240];
     std::vector<unsigned char> buf((size_t)ns * 4);
     int n = 0, rc = OSE_OK;
-    size_t used = 0;
-    while (std::fread(h, 1, sizeof h, f) == sizeof h) {
+    size_t used = 0, got;
+    while ((got = std::fread(h, 1, sizeof h, f)) == sizeof h) {
         if (get_be16(h, 114) != ns || std::fread(buf.data(), 1, buf.size(), f) != buf.size()) {
             rc = OSE_EINVAL;
             break;
@@ -285,6 +288,8 @@ int oseh_read_segy(const char* path, float* traces, size_t cap, int* nrec, int* 
         used += (size_t)ns;
         ++n;
     }
+    // a file that ends inside a trace header is cut short, not complete
+    if (rc == OSE_OK && got != sizeof h && got != 0) rc = OSE_EINVAL;
     std::fclose(f);
     if (nrec) *nrec = n;
     if (nt) *nt = ns;
