@@ -24,7 +24,7 @@ void oseh_ricker(double* w, int nt, double fm, double dt, double amp, int delay)
 {
     // ose_3d_kernel.cu:3-13: tmp = pi fm ((i - delay) dt - 1/fm); w = amp (1 - 2 tmp^2) e^{-tmp^2}
     for (int i = 0; i < nt; ++i) {
-        double t = M_PI * fm * ((double)(i - delay) * dt - 1.0 / fm);
+        double t = M_PI * fm * (((double)i - (double)delay) * dt - 1.0 / fm);
         t *= t;
         w[i] = amp * (1.0 - 2.0 * t) * std::exp(-t);
     }
@@ -118,14 +118,19 @@ double oseh_spectral_radius(double vmax, double dx, double dy, double dz)
     return vmax * M_PI * std::sqrt(1.0 / (dx * dx) + 1.0 / (dy * dy) + 1.0 / (dz * dz));
 }
 
+// R dt beyond this has no usable series order (K <= 64 holds up to ~50)
+constexpr double kZMax = 1e4;
+
 int oseh_order_reference(double z)
 {
     // ose_3d.cu:197: M = (int)dtR + 6; the loop runs to order M + 1
+    if (!(z >= 0.0) || z > kZMax) return -1;
     return (int)z + 6 + 1;
 }
 
 int oseh_order_tail(double z, double tol)
 {
+    if (!(z >= 0.0) || z > kZMax || !(tol > 0.0)) return -1;
     const int nmax = (int)z + 80;
     std::vector<double> J((size_t)nmax + 1);
     if (oseh_bessel_all(z, nmax, J.data())) return -1;

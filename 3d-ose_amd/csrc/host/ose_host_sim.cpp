@@ -3,6 +3,8 @@
 // key=value pairs and presets, model and shot construction, and the time
 // loop through include/ose_api.h.
 #include <algorithm>
+#include <cerrno>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -75,15 +77,37 @@ struct oseh_sim {
         auto it = p.find(k);
         return it == p.end() ? d : it->second;
     }
+    // Numeric keys parse strictly: a value that is not a whole finite number
+    // (in int range for geti) gives the default and is remembered in `bad`,
+    // which oseh_sim_build reports as an error.
+    mutable std::string bad;
     double getd(const std::string& k, double d) const
     {
         auto it = p.find(k);
-        return it == p.end() ? d : std::atof(it->second.c_str());
+        if (it == p.end()) return d;
+        const char* c = it->second.c_str();
+        char* end = nullptr;
+        errno = 0;
+        const double v = std::strtod(c, &end);
+        if (end == c || *end || errno == ERANGE || !std::isfinite(v)) {
+            if (bad.empty()) bad = k + "=" + it->second;
+            return d;
+        }
+        return v;
     }
     int geti(const std::string& k, int d) const
     {
         auto it = p.find(k);
-        return it == p.end() ? d : std::atoi(it->second.c_str());
+        if (it == p.end()) return d;
+        const char* c = it->second.c_str();
+        char* end = nullptr;
+        errno = 0;
+        const long v = std::strtol(c, &end, 10);
+        if (end == c || *end || errno == ERANGE || v < INT_MIN || v > INT_MAX) {
+            if (bad.empty()) bad = k + "=" + it->second;
+            return d;
+        }
+        return (int)v;
     }
 };
 
@@ -174,12 +198,14 @@ int build_specs(oseh_sim* s)
                     m.shots.push_back({(int)(m.nx * (0.2 + 0.2 * ix)), (int)(m.ny * (0.35 + 0.3 * iy)), 0});
         } else if (s->p.count("srcgrid")) {
             const int g = s->geti("srcgrid", 1);
+            if (g < 1 || g > 256) return s->fail(OSE_EINVAL, "srcgrid must be in [1, 256]");
             for (int iy = 0; iy < g; ++iy)
                 for (int ix = 0; ix < g; ++ix)
                     m.shots.push_back({(int)((ix + 0.5) * m.nx / g), (int)((iy + 0.5) * m.ny / g), s->geti("sz", 0)});
         } else {
             // shots on a line: (sx + is*jsx, sy + is*jsy, sz + is*jsz), sg_init_host ose_3d_lib.cu:598-608
             const int ns = s->geti("nshot", 1);
+            if (ns < 1 || ns > 65536) return s->fail(OSE_EINVAL, "nshot must be in [1, 65536]");
             for (int k = 0; k < ns; ++k)
                 m.shots.push_back({s->geti("sx", m.nx / 2) + k * s->geti("jsx", 1),
                                    s->geti("sy", m.ny / 2) + k * s->geti("jsy", 1),
@@ -202,6 +228,8 @@ int build_specs(oseh_sim* s)
 // reference's grid), and says so on stderr and in the info record.
 int axis_pads(oseh_sim* s, const char* axis, int n, int b, int& before, int& after, int& nb, std::string& adjusted)
 {
+    if (n < 1 || n > 4096 || b < 0 || b > 4096)
+        return s->fail(OSE_EINVAL, std::string(axis) + ": interior cells in [1, 4096] and pad cells in [0, 4096]");
     before = b;
     after = b;
     nb = before + n + after;
@@ -252,8 +280,12 @@ int build_model(oseh_sim* s, const ModelSpec& spec, BuiltModel& m)
     m.v.resize((size_t)m.nxb * m.nyb * m.nzb);
     oseh_pad_velocity(vi.data(), spec.nx, spec.ny, spec.nz, m.bl, m.br, m.bf, m.bb, m.bu, m.bd, m.v.data());
     m.vmax = *std::max_element(m.v.begin(), m.v.end());   // find_max, ose_3d_lib.cu:544-555
-    m.R = oseh_spectral_radius(m.vmax, s->getd("dx", 10), s->getd("dy", 10), s->getd("dz", 10));
-    m.z = m.R * s->getd("dt", 1e-3);
+    const double dx = s->getd("dx", 10), dy = s->getd("dy", 10), dz = s->getd("dz", 10), dt = s->getd("dt", 1e-3);
+    if (!(dx > 0) || !(dy > 0) || !(dz > 0) || !(dt > 0) || !std::isfinite(dx * dy * dz * dt))
+        return s->fail(OSE_EINVAL, "dx, dy, dz and dt must be positive");
+    if (!(m.vmax > 0) || !std::isfinite(m.vmax)) return s->fail(OSE_EINVAL, "velocities must be positive and finite");
+    m.R = oseh_spectral_radius(m.vmax, dx, dy, dz);
+    m.z = m.R * dt;
     const std::string ko = s->get("korder", "tail");
     if (ko == "ref")
         m.K = oseh_order_reference(m.z);
@@ -401,10 +433,35 @@ static int build_receivers(oseh_sim* s)
     return OSE_OK;
 }
 
+static int sim_build(oseh_sim* s);
+
 int oseh_sim_build(oseh_sim* s)
 {
     if (!s) return OSE_EINVAL;
     if (!s->models.empty()) return OSE_OK;
+    int rc;
+    try {
+        rc = sim_build(s);
+    } catch (const std::bad_alloc&) {
+        s->models.clear();
+        return s->fail(OSE_ENOMEM, "out of host memory building the models");
+    } catch (const std::exception& e) {
+        s->models.clear();
+        return s->fail(OSE_EINVAL, std::string("model build failed: ") + e.what());
+    }
+    if (rc) {
+        s->models.clear();
+        return rc;
+    }
+    if (!s->bad.empty()) {
+        s->models.clear();
+        return s->fail(OSE_EINVAL, "parameter " + s->bad + " is not a number in range");
+    }
+    return OSE_OK;
+}
+
+static int sim_build(oseh_sim* s)
+{
     int rc;
     if ((rc = build_specs(s))) return rc;
     // source wavelets (Alloc_wavelet, ose_3d_lib.cu:557-586)
@@ -545,6 +602,7 @@ int oseh_sim_setup(oseh_sim* s)
             if ((rc = ose_set_boundary_net(s->ctx, m.id, par.data(), par.size(), s->getd("guard", 4.0), replace)))
                 return s->fail(rc, std::string("ose_set_boundary_net: ") + ose_last_error(s->ctx));
     }
+    if (!s->bad.empty()) return s->fail(OSE_EINVAL, "parameter " + s->bad + " is not a number in range");
     return OSE_OK;
 }
 

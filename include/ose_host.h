@@ -40,9 +40,11 @@ int oseh_bessel_all(double x, int nmax, double* J);
 int oseh_coefficients(double z, int K, double* coef);
 /* R = vmax * pi * sqrt(dx^-2 + dy^-2 + dz^-2)  (ose_3d.cu:194).            */
 double oseh_spectral_radius(double vmax, double dx, double dy, double dz);
-/* Reference order rule: M = (int)(R dt) + 6, K = M + 1 (ose_3d.cu:197).    */
+/* Reference order rule: M = (int)(R dt) + 6, K = M + 1 (ose_3d.cu:197).
+ * -1 for z = R dt outside [0, 1e4] (or not a number).                      */
 int oseh_order_reference(double z);
-/* Smallest K with 2 sum_{k>K} |J_k(z)| < tol.                              */
+/* Smallest K with 2 sum_{k>K} |J_k(z)| < tol; -1 for z outside [0, 1e4]
+ * or tol <= 0.                                                              */
 int oseh_order_tail(double z, double tol);
 
 /* ---- taper ----------------------------------------------------------- */

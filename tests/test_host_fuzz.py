@@ -97,3 +97,29 @@ def test_trace_files_cut_inside_a_header_are_rejected(L, tmp_path):
             rc = read(p.encode(), fp, out.size, ctypes.byref(a), ctypes.byref(c), ctypes.byref(dt))
             assert (rc == 0) == ok, (write.__name__, cut, rc)
             assert a.value == ntr
+
+
+@pytest.mark.parametrize("params", [
+    "nx=-5", "nz=0", "boundary=-3", "nx=5000", "dt=0", "dt=-1e-3", "dx=nan", "dz=inf", "dt=abc", "nt=0",
+    "nt=12x", "nshot=100000000", "nshot=0", "srcgrid=100000", "srcgrid=0", "korder=ref dt=1e9", "ktol=0",
+    "seed=99999999999", "model=const v0=0", "model=const v0=-100",
+])
+def test_bad_parameters_are_rejected(params):
+    """Values that are not numbers, out-of-range sizes, shot counts, steps
+    and spacings end in an error from the host's build, never in a crash or
+    an unbounded allocation (found by random parameter strings under
+    AddressSanitizer and UndefinedBehaviorSanitizer)."""
+    from ose_amd.host import Simulation
+    sim = Simulation(f"nx=12 ny=10 nz=8 nt=6 boundary=4 {params}")
+    try:
+        with pytest.raises(Exception):
+            sim.build()
+    finally:
+        sim.close()
+
+
+def test_extreme_delay_builds():
+    """delay is any int: the wavelet's (i - delay) is formed in double."""
+    from ose_amd.host import Simulation
+    sim = Simulation("nx=12 ny=10 nz=8 nt=6 boundary=4 delay=-2147483648").build()
+    sim.close()
