@@ -786,7 +786,9 @@ int enqueue_model_orders(ose_ctx* c, Model& m, int* d_it, int s0, int nshot, int
         a.Xout = X;
         const AccPlan& pl = m.acc_plan[j];
         int mode, kind;
-        double bytes = x_pass_bytes(j, K, pl.touch, pl.ea >= 0) * cells;
+        // the shots of a launch share v/R, read once (x_block keeps a line
+        // block's shots together on one XCD, so the others find it in L2)
+        double bytes = x_pass_bytes(j, K, pl.touch, pl.ea >= 0) * cells - 4.0 * (double)(cells - m.ncell);
         if (j == 0) {
             mode = pl.touch ? XM_ORDER0 : XM_ORDER0N;
             kind = PK_X0;

@@ -264,13 +264,13 @@ __global__ __launch_bounds__(512) void k_gen_xpass(XArgs a)
     const int N = a.g.N, P = a.g.W, TP = a.g.TP;
     const int l = threadIdx.x / TP;
     const int t = threadIdx.x % TP;
-    // XCD-contiguous line blocks, as k_xpass
-    const unsigned bx = blockIdx.x, bq = gridDim.x / 8, br = gridDim.x % 8;
-    const long line = (long)((bx % 8) * bq + min(bx % 8, br) + bx / 8) * P + l;
+    // XCD-contiguous line blocks, shots fastest, as k_xpass
+    const XBlock xb = x_block();
+    const long line = (long)xb.blk * P + l;
     const bool okl = line < a.nlines;
     const int y = okl ? (int)(line % a.ny) : 0;
     const int z = okl ? (int)(line / a.ny) : 0;
-    const long lbase = (long)blockIdx.y * a.shot_stride + line * N;
+    const long lbase = (long)xb.shot * a.shot_stride + line * N;
     const long vbase = line * N;
     // the root table, then P lines
     float2* L = gen_lds + N + (long)l * N;
@@ -311,7 +311,7 @@ __global__ __launch_bounds__(512) void k_gen_xpass(XArgs a)
         // written by this thread only
         const float2* win = a.Win + lbase;
         float2* acc = a.A + lbase;
-        float* tr = rec ? a.traces + (long)blockIdx.y * a.trace_shot_stride +
+        float* tr = rec ? a.traces + (long)xb.shot * a.trace_shot_stride +
                               ((long)itv * a.rec_ny + (y - a.rec_y0)) * a.rec_nx
                         : nullptr;
         for (int x = t; x < N; x += TP) {

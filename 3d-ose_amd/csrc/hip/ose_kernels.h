@@ -701,6 +701,28 @@ __device__ __forceinline__ float2 x_update(const XArgs& a, float2 lw, int x, flo
     return wn;
 }
 
+// Line block and shot of an x-pass workgroup (grid: line blocks x shots).
+// The hardware deals workgroups round-robin over 8 XCDs by linear id, so id
+// runs on XCD id % 8; that XCD takes the contiguous range of logical blocks
+// starting at (id % 8) q + min(id % 8, r) (q = blocks / 8, r = blocks % 8)
+// instead of every 8th one.  The shot is the fastest logical index: the
+// shots of one line block run back to back on one XCD, so the model's v/R
+// lines they share come from that XCD's L2 after the first shot's read.
+struct XBlock {
+    unsigned blk, shot;
+};
+
+__device__ __forceinline__ XBlock x_block()
+{
+    const unsigned ns = gridDim.y, n = gridDim.x * ns;
+    const unsigned id = blockIdx.x + gridDim.x * blockIdx.y;
+    const unsigned q = n / 8, r = n % 8, xc = id % 8;
+    const unsigned lg = xc * q + min(xc, r) + id / 8;
+    if (ns == 1) return {lg, 0u};
+    const unsigned blk = lg / ns;
+    return {blk, lg - blk * ns};
+}
+
 template <int N1, int N2, int P, int MODE, int SPLIT, int TWB>
 __global__ __launch_bounds__((XCfg<N1, N2, P, SPLIT>::THREADS))
 void k_xpass(XArgs a)
@@ -710,16 +732,12 @@ void k_xpass(XArgs a)
     extern __shared__ float2 lds[];
     const int l = threadIdx.x / C::NT;
     const int t = threadIdx.x % C::NT;
-    // Line block: the hardware deals workgroups round-robin over 8 XCDs, so
-    // block b of a shot runs on the XCD of b % 8; that XCD takes the
-    // contiguous range starting at (b % 8) q + min(b % 8, r) (q = blocks / 8,
-    // r = blocks % 8) instead of every 8th block.
-    const unsigned bx = blockIdx.x, bq = gridDim.x / 8, br = gridDim.x % 8;
-    const long line = (long)((bx % 8) * bq + min(bx % 8, br) + bx / 8) * P + l;
+    const XBlock xb = x_block();
+    const long line = (long)xb.blk * P + l;
     const bool okl = line < a.nlines;
     const int y = okl ? (int)(line % a.ny) : 0;
     const int z = okl ? (int)(line / a.ny) : 0;
-    const long lbase = (long)blockIdx.y * a.shot_stride + line * N;
+    const long lbase = (long)xb.shot * a.shot_stride + line * N;
     const long vbase = line * N;
     // this line's exchange area: LINE float2 elements, or LINE floats when split
     void* L = SPLIT ? (void*)(reinterpret_cast<float*>(lds) + l * C::LINE) : (void*)(lds + l * C::LINE);
@@ -771,7 +789,7 @@ void k_xpass(XArgs a)
         for (int n1 = 0; n1 < N1; ++n1) wout[N2 * n1 + t] = u[n1];
         if ((MODE == XM_LAST || MODE == XM_LASTA) && rec) {
             // receivers: Im(u) on the plane z = rec_z
-            float* tr = a.traces + (long)blockIdx.y * a.trace_shot_stride + ((long)itv * a.rec_ny + (y - a.rec_y0)) * a.rec_nx;
+            float* tr = a.traces + (long)xb.shot * a.trace_shot_stride + ((long)itv * a.rec_ny + (y - a.rec_y0)) * a.rec_nx;
 #pragma unroll
             for (int n1 = 0; n1 < N1; ++n1) {
                 const int x = N2 * n1 + t;

@@ -38,11 +38,15 @@ PASS_BYTES_DOC = {
     "fwd_y": "16 B/cell (read X, write S; in place on X for batches under 1 GiB per field)",
     "z_symbol": "16 B/cell (read S, write X; in place on X for batches under 1 GiB per field)",
     "inv_y": "16 B/cell (read X, write S; in place on X for batches under 1 GiB per field)",
-    "x_epilogue": "52 B/cell (read S, W_{j-1}, v/R, A; write X_{j+1}, W_{j+1}, A)",
-    "x_acc_first": "44 B/cell (read S, W_{j-1}, v/R; write X_{j+1}, W_{j+1}, A: the order that starts A)",
-    "x_recur": "36 B/cell (read S, W_{j-1}, v/R; write X_{j+1}, W_{j+1}; accumulator not touched)",
-    "x_first": "28 B/cell (read S, v/R; write X_1, W_1); 44 where it starts the accumulator (+W_0 read, A write)",
-    "x_last": "44 B/cell (read S, W_{K-2}, v/R, A; write X_0, u)",
+    "x_epilogue": "48 B/cell (read S, W_{j-1}, A; write X_{j+1}, W_{j+1}, A) + 4 B per model cell (v/R, read once "
+                  "for all the launch's shots)",
+    "x_acc_first": "40 B/cell (read S, W_{j-1}; write X_{j+1}, W_{j+1}, A: the order that starts A) + 4 B per model "
+                   "cell (v/R)",
+    "x_recur": "32 B/cell (read S, W_{j-1}; write X_{j+1}, W_{j+1}; accumulator not touched) + 4 B per model cell "
+               "(v/R)",
+    "x_first": "24 B/cell (read S; write X_1, W_1); 40 where it starts the accumulator (+W_0 read, A write); + 4 B "
+               "per model cell (v/R)",
+    "x_last": "40 B/cell (read S, W_{K-2}, A; write X_0, u) + 4 B per model cell (v/R)",
     "inject": "16 B per x cell of the source line",
 }
 HBM_PEAK_GBS = 8000.0
@@ -267,17 +271,32 @@ def accuracy_leg(nsteps=20):
     return out
 
 
-def closed_form_leg(nsteps, chunk=250):
-    """One 864^3 shot of the 768^3 workload's geometry (K = 11, its kernels
-    and launch shapes), constant v and no taper, from a sum of four Fourier
-    modes; after nsteps each mode must carry exp(-i n dt v |k|) to within the
-    series tail and FP32 rounding (tests/_planewave.py).  Steps run in
-    chunks with a progress line each.  Returns (record, within bounds)."""
+def closed_form_leg(nsteps, config="c4", chunk=250):
+    """Closed-form check over a long horizon: constant v = 4500 m/s, no
+    taper, each shot a sum of four Fourier modes; after nsteps each mode must
+    carry exp(-i n dt v |k|) to within the series tail and FP32 rounding
+    (tests/_planewave.py).  c4: one 864^3 shot of the 768^3 workload's
+    geometry (K = 11, its kernels and launch shapes); c2: every shot of the
+    batched workload (16 shots of 600^2 x 480 and four uneven models, two
+    lanes), each from its own modes.  Steps run in chunks with a progress
+    line each.  Returns (record, within bounds)."""
     import numpy as np
     from ose_amd.host import Simulation
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _planewave as pw
     v = 4500.0
+    if config == "c2":
+        res = pw.const_batch("c2", v, nsteps, lanes=2, chunk=min(chunk, 100),
+                             log=lambda m: print(f"[bench] closed form:{m}", file=sys.stderr, flush=True))
+        worst = {q: max(float(r[q]) for r in res) for q in ("amp", "phase", "rest")}
+        rec = {"workload": f"16 shots of 600^2 x 480 + 4 uneven models, two lanes, {nsteps} steps",
+               "shots": [{"shot": r["shot"], "padded": r["padded"], "K": r["K"],
+                          **{q: float(r[q]) for q in ("amp", "amp_bound", "phase", "phase_bound", "rest", "rest_bound")}}
+                         for r in res],
+               **{f"max_{q}": x for q, x in worst.items()}}
+        return rec, all(pw.within(r) for r in res)
+    if config != "c4":
+        raise SystemExit("--closed-form-steps runs the c4 or the c2 workload")
     sim = Simulation(f"preset=c4 model=const v0={v} taper=none amp=0 nt={nsteps} max_shots=1").setup()
     try:
         m = sim.model(0)
@@ -501,7 +520,7 @@ def main():
         print(json.dumps(shape_sweep(f"{params} {args.extra}")))
         return
     if args.closed_form_steps > 0:
-        rec, ok = closed_form_leg(args.closed_form_steps)
+        rec, ok = closed_form_leg(args.closed_form_steps, args.config)
         print(json.dumps(dict(rec, within_bounds=ok)))
         return
     prof_steps = max(2, min(args.steps, 4))

@@ -98,3 +98,96 @@ def check_modes(re, im, modes, amps, nsteps, dt, shape, d, v, tail, z):
     return dict(amp=amp_err, amp_bound=nsteps * (tail + 2e-7) + 2e-5,
                 phase=ph_err, phase_bound=nsteps * (tail + 6e-7 * z) + 2e-5,
                 rest=rest, rest_bound=1e-4 + 2e-6 * np.sqrt(nsteps))
+
+
+D = (10.0, 10.0, 10.0)
+DT = 1e-3
+
+
+def add_sim_model(ctx, sim, mi, v=None):
+    """Model mi of a host Simulation's geometry added to a Context (v in
+    place of the preset's velocity when given).  Returns (id, model info)."""
+    m = sim.model(mi)
+    nx, ny, nz, bl, br, bf, bb, bu, bd, nxb, nyb, nzb = m["dims"]
+    return ctx.add_model(m["v"] if v is None else v, interior=(nx, ny, nz), origin=(bl, bf, bu), d=D, dt=DT,
+                         R=m["R"], coef=m["coef"], taper=(m["gx"], m["gy"], m["gz"]), rec_z=m["rec_z"]), m
+
+
+def series_tail(z, K):
+    """2 sum_{k>K} |J_k(z)|: what the truncated series drops per step."""
+    import scipy.special as sp
+    return 2.0 * sum(abs(sp.jv(k, z)) for k in range(K + 1, K + 40))
+
+
+def shot_modes(k, shape):
+    """Four modes per shot, different for every shot: a low one, two high
+    ones and the Nyquist corner (the top of the operator's spectrum)."""
+    nz, ny, nx = shape
+    modes = [(3 + k, 5 + 2 * k, 7 + k),
+             (nx // 3 + 13 * k, ny // 2 - 7 * k - 1, nz // 5 + 11 * k),
+             (nx - 40 - 9 * k, 31 + 17 * k, nz // 2 - 3 - 5 * k),
+             (nx // 2, ny // 2, nz // 2)]
+    amps = [1.0, 0.5 - 0.25j, 0.3j + 0.05 * k, 0.2 * (-1) ** k]
+    return modes, amps
+
+
+def const_batch(preset, v, nsteps, lanes=None, extra="", chunk=0, log=print):
+    """Step every shot of a preset's geometry on a constant velocity with no
+    taper, each from its own sum of modes (shot_modes), and check each shot
+    against the closed form.  chunk > 0 runs the steps in chunks of that
+    many with a progress line each.  Returns one check_modes dict per shot,
+    with the shot's index, padded extents and K added."""
+    import ose_amd
+    from ose_amd.host import Simulation
+    sim = Simulation(f"preset={preset} model=const v0={v} taper=none amp=0 nt={nsteps} {extra}").build()
+    results = []
+    try:
+        with ose_amd.Context(0) as c:
+            if lanes:
+                c.set_streams(lanes)
+            shots = []
+            models = []
+            for mi in range(sim.nmodels()):
+                m = sim.model(mi)
+                nxb, nyb, nzb = m["dims"][9:12]
+                vc = np.full((nzb, nyb, nxb), v, dtype=np.float32)
+                mid, mm = add_sim_model(c, sim, mi, vc)
+                assert np.all(mm["gx"] == 1) and np.all(mm["gz"] == 1)
+                models.append((mid, mm))
+            for k in range(sim.nshots()):
+                sh = sim.shot(k)
+                mid, mm = models[sh["model"]]
+                shots.append((c.add_shot(mid, tuple(sh["src"]), sh["s"], sh["h"]), mm, k))
+            for sid, mm, k in shots:
+                shape = tuple(mm["v"].shape)
+                modes, amps = shot_modes(k, shape)
+                re, im = modes_field(shape, modes, amps)
+                c.set_wavefield(sid, re, im)
+                del re, im
+            done = 0
+            while done < nsteps:
+                n = nsteps - done if chunk <= 0 else min(chunk, nsteps - done)
+                c.run(n)
+                c.sync()
+                done += n
+                if chunk > 0:
+                    log(f"  {done}/{nsteps} steps")
+            for sid, mm, k in shots:
+                shape = tuple(mm["v"].shape)
+                modes, amps = shot_modes(k, shape)
+                re, im = c.wavefield(sid)
+                assert np.isfinite(re).all() and np.isfinite(im).all()
+                z = mm["R"] * DT
+                r = check_modes(re, im, modes, amps, nsteps, DT, shape, D, v, series_tail(z, mm["K"]), z)
+                del re, im
+                log(f"  shot {k} {shape[::-1]} K={mm['K']}: amp {r['amp']:.2e} (bound {r['amp_bound']:.2e}) "
+                    f"phase {r['phase']:.2e} (bound {r['phase_bound']:.2e}) rest {r['rest']:.2e} "
+                    f"(bound {r['rest_bound']:.2e})")
+                results.append(dict(r, shot=k, padded=list(shape[::-1]), K=int(mm["K"])))
+    finally:
+        sim.close()
+    return results
+
+
+def within(r):
+    return bool(r["amp"] < r["amp_bound"] and r["phase"] < r["phase_bound"] and r["rest"] < r["rest_bound"])

@@ -26,101 +26,21 @@ import os
 import numpy as np
 import pytest
 
-from _planewave import check_modes, modes_field
+import _planewave as pw
 
 # each test moves tens of GB through the host for its checks: minutes, not seconds
 pytestmark = [pytest.mark.gpu, pytest.mark.timeout(900)]
-D = (10.0, 10.0, 10.0)
-DT = 1e-3
-
-
-def _digest(a):
-    try:
-        import xxhash
-        return xxhash.xxh3_128_hexdigest(memoryview(np.ascontiguousarray(a)).cast("B"))
-    except ImportError:
-        import hashlib
-        return hashlib.blake2b(memoryview(np.ascontiguousarray(a)).cast("B")).hexdigest()
-
-
-def _add_sim_model(ctx, sim, mi, v=None):
-    m = sim.model(mi)
-    nx, ny, nz, bl, br, bf, bb, bu, bd, nxb, nyb, nzb = m["dims"]
-    return ctx.add_model(m["v"] if v is None else v, interior=(nx, ny, nz), origin=(bl, bf, bu), d=D, dt=DT,
-                         R=m["R"], coef=m["coef"], taper=(m["gx"], m["gy"], m["gz"]), rec_z=m["rec_z"]), m
-
-
-def _tail(z, K):
-    import scipy.special as sp
-    return 2.0 * sum(abs(sp.jv(k, z)) for k in range(K + 1, K + 40))
-
-
-def _shot_modes(k, shape):
-    """Four modes per shot, different for every shot: a low one, two high
-    ones and the Nyquist corner (the top of the operator's spectrum)."""
-    nz, ny, nx = shape
-    modes = [(3 + k, 5 + 2 * k, 7 + k),
-             (nx // 3 + 13 * k, ny // 2 - 7 * k - 1, nz // 5 + 11 * k),
-             (nx - 40 - 9 * k, 31 + 17 * k, nz // 2 - 3 - 5 * k),
-             (nx // 2, ny // 2, nz // 2)]
-    amps = [1.0, 0.5 - 0.25j, 0.3j + 0.05 * k, 0.2 * (-1) ** k]
-    return modes, amps
+D, DT = pw.D, pw.DT
+_add_sim_model = pw.add_sim_model
 
 
 def _const_batch(preset, v, nsteps, lanes=None, extra=""):
-    """Step every shot of a preset's geometry on a constant velocity with no
-    taper from a different sum of modes; returns the closed-form errors."""
-    import ose_amd
-    from ose_amd.host import Simulation
-    sim = Simulation(f"preset={preset} model=const v0={v} taper=none amp=0 nt={nsteps} {extra}").build()
-    results = []
-    try:
-        with ose_amd.Context(0) as c:
-            if lanes:
-                c.set_streams(lanes)
-            shots = []
-            models = []
-            for mi in range(sim.nmodels()):
-                m = sim.model(mi)
-                nxb, nyb, nzb = m["dims"][9:12]
-                vc = np.full((nzb, nyb, nxb), v, dtype=np.float32)
-                mid, mm = _add_sim_model(c, sim, mi, vc)
-                assert np.all(mm["gx"] == 1) and np.all(mm["gz"] == 1)
-                models.append((mid, mm))
-            for k in range(sim.nshots()):
-                sh = sim.shot(k)
-                mid, mm = models[sh["model"]]
-                shots.append((c.add_shot(mid, tuple(sh["src"]), sh["s"], sh["h"]), mm, k))
-            for sid, mm, k in shots:
-                shape = tuple(mm["v"].shape)
-                modes, amps = _shot_modes(k, shape)
-                re, im = modes_field(shape, modes, amps)
-                c.set_wavefield(sid, re, im)
-                del re, im
-            c.run(nsteps)
-            c.sync()
-            for sid, mm, k in shots:
-                shape = tuple(mm["v"].shape)
-                modes, amps = _shot_modes(k, shape)
-                re, im = c.wavefield(sid)
-                assert np.isfinite(re).all() and np.isfinite(im).all()
-                z = mm["R"] * DT
-                r = check_modes(re, im, modes, amps, nsteps, DT, shape, D, v, _tail(z, mm["K"]), z)
-                del re, im
-                print(f"  shot {k} {shape[::-1]} K={mm['K']}: amp {r['amp']:.2e} (bound {r['amp_bound']:.2e}) "
-                      f"phase {r['phase']:.2e} (bound {r['phase_bound']:.2e}) rest {r['rest']:.2e} "
-                      f"(bound {r['rest_bound']:.2e})", flush=True)
-                results.append(r)
-    finally:
-        sim.close()
-    return results
+    return pw.const_batch(preset, v, nsteps, lanes=lanes, extra=extra, log=lambda m: print(m, flush=True))
 
 
 def _assert_modes(results):
     for r in results:
-        assert r["amp"] < r["amp_bound"]
-        assert r["phase"] < r["phase_bound"]
-        assert r["rest"] < r["rest_bound"]
+        assert pw.within(r), r
 
 
 def test_c4_batch_plane_waves_closed_form():
