@@ -325,6 +325,52 @@ def closed_form_leg(nsteps, config="c4", chunk=250):
     return rec, bool(r["amp"] < r["amp_bound"] and r["phase"] < r["phase_bound"] and r["rest"] < r["rest_bound"])
 
 
+def learned_horizon_leg(nsteps, chunk=250):
+    """The 768^3 workload's learned leg over a long horizon: one 784^3 shot
+    (8-cell pad, committed network, salt model, Ricker source, surface
+    receivers) for nsteps steps, in chunks with a progress line each.  After
+    each chunk: total field energy, finiteness, the network's state.  The
+    guard must hold, the traces stay finite, and once the source has stopped
+    (step 500) the energy must not grow from one sample to the next beyond
+    FP32 rounding, 2e-6 per step (the propagator is unitary and the
+    network's gain passive).  Returns
+    (record, all held)."""
+    import numpy as np
+    from ose_amd import boundary
+    from ose_amd.host import Simulation
+    sim = Simulation(f"preset=c4 boundary=8 max_shots=1 nt={nsteps} bnet={boundary.WEIGHTS}").setup()
+    rows = []
+    t0 = time.time()
+    done = 0
+    try:
+        while done < nsteps:
+            n = min(chunk, nsteps - done)
+            sim.run(n)
+            done += n
+            re, im = sim.wavefield(0)
+            fin = bool(np.isfinite(re).all() and np.isfinite(im).all())
+            e = float(np.sum(re.astype(np.float64) ** 2) + np.sum(im.astype(np.float64) ** 2))
+            del re, im
+            st = sim.boundary_status(0)
+            rows.append({"step": done, "energy": e, "finite": fin, "active": bool(st["active"]),
+                         "tripped": bool(st["tripped"])})
+            print(f"[bench] learned horizon: {json.dumps(rows[-1])} {time.time() - t0:.0f} s", file=sys.stderr,
+                  flush=True)
+        tr = sim.traces(0, 0, nsteps)
+    finally:
+        sim.close()
+    after = [r["energy"] for r in rows if r["step"] >= 500]
+    grows = max((b / a for a, b in zip(after, after[1:])), default=0.0)
+    rec = {"workload": f"one 784^3 shot of the 768^3 workload, learned 8-cell pad, {nsteps} steps",
+           "samples": rows, "traces_finite": bool(np.isfinite(tr).all()), "trace_max": float(np.abs(tr).max()),
+           "guard_held": all(not r["tripped"] for r in rows), "network_active": rows[-1]["active"],
+           "max_energy_ratio_between_samples_after_step_500": grows,
+           "energy_ratio_step_500_to_end": after[-1] / after[0] if after else None}
+    ok = (rec["traces_finite"] and all(r["finite"] for r in rows) and rec["guard_held"] and rec["network_active"]
+          and grows <= 1.0 + 2e-6 * chunk)       # FP32 rounding per step, as the GPU tests allow
+    return rec, bool(ok)
+
+
 def snapshot_leg(params, steps, every, warmup=2):
     """Cost of snapshot streaming on the running workload, wall clock around
     the steps and the final drain: steps without snapshots; with snapshots
@@ -500,8 +546,12 @@ def main():
     ap.add_argument("--shape-sweep", action="store_true",
                     help="instead of the throughput line: every strided launch shape's pass durations")
     ap.add_argument("--closed-form-steps", type=int, default=0, metavar="N",
-                    help="instead of the throughput line: the closed-form mode check on one 864^3 shot of the "
-                         "768^3 workload's geometry after N steps")
+                    help="instead of the throughput line: the closed-form mode check after N steps on one 864^3 "
+                         "shot of the 768^3 workload's geometry (--config c4) or every shot of the batched "
+                         "workload (--config c2)")
+    ap.add_argument("--learned-horizon", type=int, default=0, metavar="N",
+                    help="instead of the throughput line: the learned leg (one 784^3 shot, 8-cell pad) over N "
+                         "steps with energy samples")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run one plain device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -522,6 +572,10 @@ def main():
     if args.closed_form_steps > 0:
         rec, ok = closed_form_leg(args.closed_form_steps, args.config)
         print(json.dumps(dict(rec, within_bounds=ok)))
+        return
+    if args.learned_horizon > 0:
+        rec, ok = learned_horizon_leg(args.learned_horizon)
+        print(json.dumps(dict(rec, held=ok)))
         return
     prof_steps = max(2, min(args.steps, 4))
     nt = args.warmup + args.steps + prof_steps
