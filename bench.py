@@ -329,7 +329,9 @@ def learned_horizon_leg(nsteps, chunk=250):
     """The 768^3 workload's learned leg over a long horizon: one 784^3 shot
     (8-cell pad, committed network, salt model, Ricker source, surface
     receivers) for nsteps steps, in chunks with a progress line each.  After
-    each chunk: total field energy, finiteness, the network's state.  The
+    each chunk: the field's energy sum |u|^2 / v (the norm the propagator
+    conserves on a variable velocity; tests/_planewave.py weighted_energy),
+    finiteness, the network's state.  The
     guard must hold, the traces stay finite, and once the source has stopped
     (step 500) the energy must not grow from one sample to the next beyond
     FP32 rounding, 2e-6 per step (the propagator is unitary and the
@@ -338,18 +340,21 @@ def learned_horizon_leg(nsteps, chunk=250):
     import numpy as np
     from ose_amd import boundary
     from ose_amd.host import Simulation
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _planewave as pw
     sim = Simulation(f"preset=c4 boundary=8 max_shots=1 nt={nsteps} bnet={boundary.WEIGHTS}").setup()
     rows = []
     t0 = time.time()
     done = 0
     try:
+        v = sim.model(0)["v"]
         while done < nsteps:
             n = min(chunk, nsteps - done)
             sim.run(n)
             done += n
             re, im = sim.wavefield(0)
             fin = bool(np.isfinite(re).all() and np.isfinite(im).all())
-            e = float(np.sum(re.astype(np.float64) ** 2) + np.sum(im.astype(np.float64) ** 2))
+            e = pw.weighted_energy(re, im, v)
             del re, im
             st = sim.boundary_status(0)
             rows.append({"step": done, "energy": e, "finite": fin, "active": bool(st["active"]),

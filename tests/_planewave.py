@@ -191,3 +191,17 @@ def const_batch(preset, v, nsteps, lanes=None, extra="", chunk=0, log=print):
 
 def within(r):
     return bool(r["amp"] < r["amp_bound"] and r["phase"] < r["phase_bound"] and r["rest"] < r["rest_bound"])
+
+
+def weighted_energy(re, im, v, chunk=32):
+    """sum |u|^2 / v over the grid, FP64, slab by slab.  The OSE operator
+    L = v |k| (v multiplying after |k|) is self-adjoint in the inner
+    product <a, b> = sum conj(a) b / v, so exp(-i dt L) conserves this norm
+    (up to the series tail) on any velocity model; sum |u|^2 is conserved
+    only where v is constant."""
+    e = 0.0
+    for z0 in range(0, re.shape[0], chunk):
+        r = re[z0:z0 + chunk].astype(np.float64)
+        i = im[z0:z0 + chunk].astype(np.float64)
+        e += float(np.sum((r * r + i * i) / v[z0:z0 + chunk]))
+    return e
