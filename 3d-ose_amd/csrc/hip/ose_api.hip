@@ -476,6 +476,21 @@ struct ose_ctx {
             return (ctx)->fail(OSE_EDEVICE, "%s failed: %s", #call, hipGetErrorString(e_));    \
     } while (0)
 
+// two timing events, destroyed on every return path
+struct EventPair {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t create()
+    {
+        const hipError_t e = hipEventCreate(&e0);
+        return e == hipSuccess ? hipEventCreate(&e1) : e;
+    }
+    ~EventPair()
+    {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+};
+
 namespace {
 
 int dev_alloc(ose_ctx* c, DevBuf& b, size_t bytes)
@@ -1002,17 +1017,14 @@ int time_variant(ose_ctx* c, const std::function<int()>& body, int reps, float* 
 {
     int rc;
     if ((rc = body())) return rc;   // warm-up
-    hipEvent_t e0, e1;
-    HIPCHK(c, hipEventCreate(&e0));
-    HIPCHK(c, hipEventCreate(&e1));
-    HIPCHK(c, hipEventRecord(e0, c->stream));
+    EventPair ev;
+    HIPCHK(c, ev.create());
+    HIPCHK(c, hipEventRecord(ev.e0, c->stream));
     for (int r = 0; r < reps && !rc; ++r) rc = body();
-    HIPCHK(c, hipEventRecord(e1, c->stream));
-    HIPCHK(c, hipEventSynchronize(e1));
+    HIPCHK(c, hipEventRecord(ev.e1, c->stream));
+    HIPCHK(c, hipEventSynchronize(ev.e1));
     float t = 0.f;
-    HIPCHK(c, hipEventElapsedTime(&t, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    HIPCHK(c, hipEventElapsedTime(&t, ev.e0, ev.e1));
     *ms = t / reps;
     return rc;
 }
@@ -1958,20 +1970,17 @@ int ose_timed_run(ose_ctx* c, int nsteps, double* ms)
         drop_graph(c);
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    hipEvent_t e0, e1;
-    HIPCHK(c, hipEventCreate(&e0));
-    HIPCHK(c, hipEventCreate(&e1));
-    HIPCHK(c, hipEventRecord(e0, c->stream));
+    EventPair ev;
+    HIPCHK(c, ev.create());
+    HIPCHK(c, hipEventRecord(ev.e0, c->stream));
     rc = ose_run(c, nsteps);
     if (rc == OSE_OK) {
-        HIPCHK(c, hipEventRecord(e1, c->stream));
-        HIPCHK(c, hipEventSynchronize(e1));
+        HIPCHK(c, hipEventRecord(ev.e1, c->stream));
+        HIPCHK(c, hipEventSynchronize(ev.e1));
         float f = 0.f;
-        HIPCHK(c, hipEventElapsedTime(&f, e0, e1));
+        HIPCHK(c, hipEventElapsedTime(&f, ev.e0, ev.e1));
         if (ms) *ms = f;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (rc) return rc;
     return collect(c);
 }
@@ -2020,17 +2029,14 @@ int ose_pattern_probe(ose_ctx* c, int model_id, int axis, int reps, double* gbs)
     if (k.lds > 65536) HIPCHK(c, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)k.lds));
     void* args[] = {&a, &n1, &n2};
     HIPCHK(c, hipLaunchKernel(fn, grid, block, args, k.lds, c->stream));   // warm-up
-    hipEvent_t e0, e1;
-    HIPCHK(c, hipEventCreate(&e0));
-    HIPCHK(c, hipEventCreate(&e1));
-    HIPCHK(c, hipEventRecord(e0, c->stream));
+    EventPair ev;
+    HIPCHK(c, ev.create());
+    HIPCHK(c, hipEventRecord(ev.e0, c->stream));
     for (int r = 0; r < reps; ++r) HIPCHK(c, hipLaunchKernel(fn, grid, block, args, k.lds, c->stream));
-    HIPCHK(c, hipEventRecord(e1, c->stream));
-    HIPCHK(c, hipEventSynchronize(e1));
+    HIPCHK(c, hipEventRecord(ev.e1, c->stream));
+    HIPCHK(c, hipEventSynchronize(ev.e1));
     float ms = 0.f;
-    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    HIPCHK(c, hipEventElapsedTime(&ms, ev.e0, ev.e1));
     *gbs = 16.0 * (double)m.ncell * nshot * reps / (ms * 1e-3) / 1e9;
     return OSE_OK;
 }
@@ -2050,17 +2056,14 @@ int ose_copy_probe(ose_ctx* c, size_t bytes, int reps, double* gbs)
     f32x4* dst = (f32x4*)b.p;
     void* args[] = {(void*)&src, (void*)&dst, (void*)&n};
     HIPCHK(c, hipLaunchKernel((const void*)&k_copy, grid, block, args, 0, c->stream));
-    hipEvent_t e0, e1;
-    HIPCHK(c, hipEventCreate(&e0));
-    HIPCHK(c, hipEventCreate(&e1));
-    HIPCHK(c, hipEventRecord(e0, c->stream));
+    EventPair ev;
+    HIPCHK(c, ev.create());
+    HIPCHK(c, hipEventRecord(ev.e0, c->stream));
     for (int r = 0; r < reps; ++r) HIPCHK(c, hipLaunchKernel((const void*)&k_copy, grid, block, args, 0, c->stream));
-    HIPCHK(c, hipEventRecord(e1, c->stream));
-    HIPCHK(c, hipEventSynchronize(e1));
+    HIPCHK(c, hipEventRecord(ev.e1, c->stream));
+    HIPCHK(c, hipEventSynchronize(ev.e1));
     float ms = 0.f;
-    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    HIPCHK(c, hipEventElapsedTime(&ms, ev.e0, ev.e1));
     *gbs = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
     return OSE_OK;
 }
