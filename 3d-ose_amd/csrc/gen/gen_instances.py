@@ -181,7 +181,7 @@ ZCHUNK_MS = (2, 3, 4, 6)
 
 def zchunk_fits(n1, n2):
     n = n1 * n2
-    lds = 8 * (n * (ZCHUNK_T + 1) + 2 * n) + 4 * n
+    lds = 8 * (n * (ZCHUNK_T + (n1 % 2 == 0)) + 2 * n) + 4 * n   # exchange pitch: XchgPitch
     return n >= 256 and n1 <= n2 and ZCHUNK_T * n2 <= 1024 and lds <= LDS_MAX
 
 

@@ -127,6 +127,20 @@ struct StridedTile {
     }
 };
 
+// Row pitch (elements) of the LDS column exchange.  A wave's exchange access
+// covers T consecutive columns of 64 / T consecutive t, whose rows step by 1
+// on one side of each exchange and by N1 on the other, so its groups of T
+// lanes start pitch * 1 or pitch * N1 elements apart.  With pitch T and odd
+// N1 the groups fall on disjoint banks on both sides (32 4-byte banks per 32
+// lanes in the half-LDS form; also with 64 banks per wave and in the 8-byte
+// form); the padded T + 1 puts two groups on overlapping banks there, about
+// twice the LDS cycles.  For even N1 no pitch serves both sides and the
+// padded row stays.
+template <int T, int N1>
+struct XchgPitch {
+    static constexpr int value = (N1 & 1) ? T : T + 1;
+};
+
 // SPLIT = 1 exchanges the real and then the imaginary parts through an LDS
 // array of floats: half the LDS per workgroup (twice the workgroups per CU
 // for the long extents) for two more barriers.
@@ -135,7 +149,7 @@ struct StridedCfg {
     static constexpr int N = N1 * N2;
     static constexpr int NT = Max<N1, N2>::value;
     static constexpr int THREADS = T * NT;
-    static constexpr int LDT = T + 1;                       // padded row
+    static constexpr int LDT = XchgPitch<T, N1>::value;     // exchange row pitch
     static constexpr size_t LDS_BYTES = (SPLIT ? sizeof(float) : sizeof(float2)) * (size_t)N * LDT;
 };
 
@@ -159,10 +173,9 @@ __device__ __forceinline__ int xrow(FROW row, int i)
     return mid + (row(i) - row(NI / 2));
 }
 
-template <int T, int SPLIT, int NW, int NR, typename FW, typename FR>
+template <int T, int LDT, int SPLIT, int NW, int NR, typename FW, typename FR>
 __device__ __forceinline__ void col_exchange(float2* v, bool wr, bool rd, FW wrow, FR rrow, float2* lds, int col)
 {
-    constexpr int LDT = T + 1;
     constexpr bool FAR = (size_t)NW * NR * LDT * (SPLIT ? sizeof(float) : sizeof(float2)) > 65536;
     if (!SPLIT) {
         if (wr) {
@@ -270,7 +283,7 @@ void k_strided(StridedArgs a)
             v[k1] = SIGN < 0 ? cmul(v[k1], w) : cmulc(v[k1], w);
         }
     }
-    col_exchange<T, SPLIT, N1, N2>(
+    col_exchange<T, C::LDT, SPLIT, N1, N2>(
         v, t < N2, t < N1, [&](int k1) { return t * N1 + k1; }, [&](int n2) { return n2 * N1 + t; }, lds, col);
     if (t < N1) {
         rdft<N2, SIGN>(v);
@@ -334,7 +347,7 @@ void k_zsym(StridedArgs a)
 #pragma unroll
         for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], twt[k1]);
     }
-    col_exchange<T, SPLIT, N1, N2>(
+    col_exchange<T, C::LDT, SPLIT, N1, N2>(
         v, t < N2, t < N1, [&](int k1) { return t * N1 + k1; }, [&](int n2) { return n2 * N1 + t; }, lds, col);
     if (t < N1) {
         rdft<N2, -1>(v);
@@ -353,7 +366,7 @@ void k_zsym(StridedArgs a)
 #pragma unroll
         for (int n2 = 1; n2 < N2; ++n2) v[n2] = cmulc(v[n2], twi[n2]);
     }
-    col_exchange<T, SPLIT, N2, N1>(
+    col_exchange<T, C::LDT, SPLIT, N2, N1>(
         v, t < N1, t < N2, [&](int n2) { return n2 * N1 + t; }, [&](int m1) { return t * N1 + m1; }, lds, col);
     if (t < N2) {
         rdft<N1, 1>(v);
@@ -380,7 +393,7 @@ struct ZChunkCfg {
     static constexpr int N = N1 * N2;
     static constexpr int NT = Max<N1, N2>::value;
     static constexpr int THREADS = T * NT;
-    static constexpr int LDT = T + 1;
+    static constexpr int LDT = XchgPitch<T, N1>::value;
     static constexpr size_t XCH = (size_t)N * LDT;   // exchange, float2
     static constexpr size_t LDS_BYTES = sizeof(float2) * (XCH + 2 * (size_t)N) + sizeof(float) * (size_t)N;
 };
@@ -453,7 +466,7 @@ void k_zsym_chunk(StridedArgs a)
 #pragma unroll
             for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], twt[k1]);
         }
-        col_exchange<T, 0, N1, N2>(
+        col_exchange<T, C::LDT, 0, N1, N2>(
             v, true, t < N1, [&](int k1) { return t * N1 + k1; }, [&](int n2) { return n2 * N1 + t; }, lds, col);
         if (t < N1) {
             rdft<N2, -1>(v);
@@ -468,7 +481,7 @@ void k_zsym_chunk(StridedArgs a)
 #pragma unroll
             for (int n2 = 1; n2 < N2; ++n2) v[n2] = cmulc(v[n2], twi[n2]);
         }
-        col_exchange<T, 0, N2, N1>(
+        col_exchange<T, C::LDT, 0, N2, N1>(
             v, t < N1, true, [&](int n2) { return n2 * N1 + t; }, [&](int m1) { return t * N1 + m1; }, lds, col);
         {
             rdft<N1, 1>(v);
