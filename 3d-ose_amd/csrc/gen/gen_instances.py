@@ -132,7 +132,7 @@ STRIDED = """    {{
         k.T = T; k.split = SPLIT; k.threads = SC::THREADS; k.lds = SC::LDS_BYTES;
         k.fn = (const void*)&k_strided<N1, N2, T, -1, SPLIT>; e.yf[e.ns] = k;
         k.fn = (const void*)&k_strided<N1, N2, T, 1, SPLIT>; e.yi[e.ns] = k;
-        k.fn = (const void*)&k_zsym<N1, N2, T, SPLIT>; e.zs[e.ns] = k;
+        k.fn = (const void*)&k_zsym<N1, N2, T, SPLIT>; k.lds = ZsymCfg<N1, N2, T, SPLIT>::LDS_BYTES; e.zs[e.ns] = k;
         ++e.ns;
     }}
 """

@@ -327,42 +327,90 @@ __global__ __launch_bounds__(T * 32) void k_strided_copy(StridedArgs a, int N1, 
 // its half-size LDS lets two 512-thread workgroups share a CU only then,
 // and the z pass needs them to cover its address translation latency.  (The
 // whole-complex form is held to one workgroup per CU by its LDS anyway.)
+//
+// The twiddles and |k_z|^2 come from LDS: a workgroup copies the twiddle
+// rows w^(t n2) (block 2 of the root table) and kz2 there before it issues
+// its pencil reads, so that the copy's reads complete first (the vector
+// memory counter completes in order) and no global read is left between
+// the pencil's arrival and the stores.  (Read from global memory at the
+// point of use, each of the three tables cost a round trip through the
+// memory queue in the middle of the arithmetic.)  With the rows w^(t n2)
+// the forward twiddle w^(k1 n2) is applied after the first exchange, by the
+// thread that gathers k1 = t: the same products the writing side would
+// form, and the same rows serve the inverse.
+template <int N1, int N2, int T, int SPLIT>
+struct ZsymCfg {
+    using SC = StridedCfg<N1, N2, T, SPLIT>;
+    static constexpr size_t TW_OFF = (SC::LDS_BYTES + 15) & ~(size_t)15;    // w^(t n2), N float2
+    static constexpr size_t KZ_OFF = TW_OFF + sizeof(float2) * (size_t)SC::N; // |k_z|^2, N floats
+    static constexpr size_t LDS_BYTES = KZ_OFF + sizeof(float) * (size_t)SC::N;
+    static_assert(LDS_BYTES <= 160 * 1024, "z pass LDS beyond a CU's 160 KiB");
+};
+
 template <int N1, int N2, int T, int SPLIT>
 __global__ __launch_bounds__((StridedCfg<N1, N2, T, SPLIT>::THREADS), (SPLIT ? 4 : 1))
 void k_zsym(StridedArgs a)
 {
     using C = StridedCfg<N1, N2, T, SPLIT>;
+    using Z = ZsymCfg<N1, N2, T, SPLIT>;
     extern __shared__ float2 lds[];
+    float2* ltw = reinterpret_cast<float2*>(reinterpret_cast<char*>(lds) + Z::TW_OFF);
+    float* lkz = reinterpret_cast<float*>(reinterpret_cast<char*>(lds) + Z::KZ_OFF);
     const int tid = threadIdx.x;
     const int col = tid % T;
     const int t = tid / T;
     const StridedTile g(a);
     const StridedAddr ad(a, g, T, col, t);
+    // the tables' and the column's reads first (indices clamped, so that no
+    // read is conditional), then the pencil's; the tables go to LDS once
+    // their reads are in, while the pencil's are still in flight
+    constexpr int NTAB = (C::N + C::THREADS - 1) / C::THREADS;
+    float2 tw_r[NTAB];
+    float kz_r[NTAB];
+#pragma unroll
+    for (int j = 0; j < NTAB; ++j) {
+        const int i = min(tid + j * C::THREADS, C::N - 1);
+        tw_r[j] = a.tw[2 * C::N + i];
+        kz_r[j] = a.kz2[i];
+    }
+    const float kxy = a.kx2[g.bx * T + ad.xc] + a.ky2[g.by];
     float2 v[C::NT];
     if (t < N2) {
 #pragma unroll
         for (int n1 = 0; n1 < N1; ++n1) v[n1] = ad.load(a.in, N2 * n1);
-        rdft<N1, -1>(v);
-        const float2* twt = a.tw + C::N + t * N1;
-#pragma unroll
-        for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], twt[k1]);
     }
+#pragma unroll
+    for (int j = 0; j < NTAB; ++j) {
+        const int i = tid + j * C::THREADS;
+        if (i < C::N) {
+            ltw[i] = tw_r[j];
+            lkz[i] = kz_r[j];
+        }
+    }
+    if (t < N2) rdft<N1, -1>(v);
     col_exchange<T, C::LDT, SPLIT, N1, N2>(
         v, t < N2, t < N1, [&](int k1) { return t * N1 + k1; }, [&](int n2) { return n2 * N1 + t; }, lds, col);
     if (t < N1) {
+        // thread t holds X1[k1 = t][n2]: twiddle w^(t n2), then the N2-point DFT
+        const float2* twr = ltw + t * N2;
+#pragma unroll
+        for (int n2 = 1; n2 < N2; ++n2) v[n2] = cmul(v[n2], twr[n2]);
         rdft<N2, -1>(v);
-        const float kxy = a.kx2[g.bx * T + ad.xc] + a.ky2[g.by];
 #pragma unroll
         for (int k2 = 0; k2 < N2; ++k2) {
             // hardware square root (1 ulp): |k| needs no correctly rounded
             // result, and the rounding-corrected sqrtf costs ~10 instructions
-            const float s = __builtin_amdgcn_sqrtf(kxy + a.kz2[t + N1 * k2]) * a.invN;
+            const float s = __builtin_amdgcn_sqrtf(kxy + lkz[t + N1 * k2]) * a.invN;
             v[k2].x *= s;
             v[k2].y *= s;
         }
         // inverse: thread m1 = t holds X[m1 + N1 m2]
         rdft<N2, 1>(v);
-        const float2* twi = a.tw + 2 * C::N + t * N2;
+        // the same rows, read again: an opaque copy of the pointer keeps the
+        // compiler from holding the forward multiply's 31 values in
+        // registers across the transforms (41 spilled VGPRs)
+        const float2* twi = twr;
+        asm("" : "+v"(twi));
 #pragma unroll
         for (int n2 = 1; n2 < N2; ++n2) v[n2] = cmulc(v[n2], twi[n2]);
     }
