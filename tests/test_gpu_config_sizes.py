@@ -34,6 +34,15 @@ D, DT = pw.D, pw.DT
 _add_sim_model = pw.add_sim_model
 
 
+def _digest(a):
+    try:
+        import xxhash
+        return xxhash.xxh3_128_hexdigest(memoryview(np.ascontiguousarray(a)).cast("B"))
+    except ImportError:
+        import hashlib
+        return hashlib.blake2b(memoryview(np.ascontiguousarray(a)).cast("B")).hexdigest()
+
+
 def _const_batch(preset, v, nsteps, lanes=None, extra=""):
     return pw.const_batch(preset, v, nsteps, lanes=lanes, extra=extra, log=lambda m: print(m, flush=True))
 
