@@ -243,7 +243,7 @@ def accuracy_leg(nsteps=20):
     """Accuracy measured in the same run, for the metric's "at seismogram
     rel-L2 <= 1e-3 vs FP64":
       fp64: the 256^3 configuration (336^3 padded, K = 12, 40-cell taper)
-            over the first 100 steps against the committed FP64 oracle
+            over the first 250 steps against the committed FP64 oracle
             seismogram (tests/golden/c1_oracle.npz, every 8th receiver);
       closed_form: one 864^3 shot of the 768^3 workload's geometry (K = 11,
             the same kernels and launch shapes), constant v and no taper,

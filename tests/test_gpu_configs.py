@@ -40,12 +40,12 @@ def test_c0_seismogram_matches_fp64_oracle():
 
 def test_c1_seismogram_matches_fp64_oracle_at_full_size():
     """256^3 layered model with anomalies, 40-cell taper, 336^3 padded, K =
-    12: the first 100 steps of the FP32 GPU run against the FP64 oracle
+    12: the first 250 steps of the FP32 GPU run against the FP64 oracle
     (every 8th receiver in x and y, tests/golden/gen_c1_oracle.py)."""
     from ose_amd.host import Simulation
     path = os.path.join(GOLD, "c1_oracle.npz")
     if not os.path.exists(path):
-        pytest.skip("c1_oracle.npz not generated (tests/golden/gen_c1_oracle.py, about an hour of FP64 CPU)")
+        pytest.skip("c1_oracle.npz not generated (tests/golden/gen_c1_oracle.py, hours of FP64 CPU)")
     g = np.load(path)
     nt = int(g["nt"])
     print(f"c1 {nt} steps", flush=True)
